@@ -1,0 +1,381 @@
+"""Reference test systems and their seeded ensembles, restated.
+
+The right-hand sides here are ordinary Python callbacks in the form ``solve``
+accepts (a tuple of scalars from constant indices), written with the same
+arithmetic order as the reference's own systems so that generated HIP code
+evaluates them the same way. The ensemble generators reproduce the reference's
+``make_scenario`` draws call for call with ``numpy.random.default_rng(seed)``,
+so the same seed gives the same trajectories.
+
+Sources (reference repository):
+  - Lorenz:        reference/systems/python/lorenz.py:136-209
+  - Robertson:     reference/systems/python/robertson.py:6-109
+  - VdP lattice:   reference/systems/python/vdp.py:224-359
+  - Kaps:          reference/systems/python/kaps.py:23-60
+  - test callbacks: tests/test_solver_options.py:142-147,
+    tests/test_sparsity.py:36-61, tests/test_sparse_direct.py:333-342,
+    tests/test_save_hook.py:23-24
+
+The benchmark ensembles that have no reference generator (the uneven stiff
+ensemble, the VdP chain with log-uniform mu) are defined here with fixed seeds.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+# --------------------------------------------------------------------------
+# Lorenz (lorenz.py)
+# --------------------------------------------------------------------------
+LORENZ_TIMES = (0.0, 5.0)
+LORENZ_Y0 = np.array([1000.0, -500.0, 500.0])
+LORENZ_PARAMS = np.array([0.5])
+
+
+def lorenz(y, t, p):
+    sigma = 10.0
+    beta = 8.0 / 3.0
+    return (
+        sigma * (y[1] - y[0]),
+        y[0] * (p[0] - y[2]) - y[1],
+        y[0] * y[1] - beta * y[2],
+    )
+
+
+def lorenz_params(size, seed=42):
+    rng = np.random.default_rng(seed)
+    return 0.5 * (1.0 + 0.05 * (2.0 * rng.random((size, 1)) - 1.0))
+
+
+def lorenz_scenario(size, seed=42, *, divergence=1.0):
+    """lorenz.py:170-209: radial ICs ``Y0 * (1 - t)``, t ~ U(0, divergence)."""
+    divergence = _check_divergence(divergence)
+    params = LORENZ_PARAMS + divergence * (lorenz_params(size, seed) - LORENZ_PARAMS)
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(0.0, divergence, size=(size, 1))
+    return LORENZ_Y0 * (1.0 - t), params
+
+
+def lorenz_rho_ensemble(size, seed=0):
+    """Tsit5 benchmark ensemble: rho ~ U[0, 28], y0 = (1, 0, 0), t in [0, 1]."""
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.0, 28.0, size=(size, 1))
+    y0 = np.tile(np.array([1.0, 0.0, 0.0]), (size, 1))
+    return y0, rho
+
+
+# --------------------------------------------------------------------------
+# Robertson (robertson.py)
+# --------------------------------------------------------------------------
+ROBERTSON_TIMES = np.array((0.0, 1e-6, 1e-2, 1e2, 1e5))
+_ALPHA = 0.99
+_EPS = 0.1
+ROBERTSON_Y0 = np.array([(1 - _EPS) * _ALPHA, _EPS, (1 - _EPS) * (1 - _ALPHA)])
+#: (k1, k3, k2): the reference orders the rates this way (robertson.py:13, 17-22).
+ROBERTSON_PARAMS = np.array([0.04, 1e4, 3e7])
+
+
+def robertson(y, t, p):
+    return (
+        -p[0] * y[0] + p[1] * y[1] * y[2],
+        p[0] * y[0] - p[1] * y[1] * y[2] - p[2] * y[1] ** 2,
+        p[2] * y[1] ** 2,
+    )
+
+
+def robertson_initial_conditions(size, seed=42, *, divergence=1.0):
+    divergence = _check_divergence(divergence)
+    if divergence == 0.0:
+        return np.broadcast_to(ROBERTSON_Y0, (size, 3)).copy()
+    rng = np.random.default_rng(seed)
+    alpha = _ALPHA * rng.uniform(0.0, 1.0, size)
+    eps_blend = rng.uniform(0.0, min(divergence / 4.0, 1.0), size)
+    eps = 10.0 ** ((1.0 - eps_blend) * np.log10(_EPS) + eps_blend * np.log10(1e-8))
+    return np.column_stack([(1 - eps) * alpha, eps, (1 - eps) * (1 - alpha)])
+
+
+def robertson_scenario(size, seed=42, *, divergence=1.0):
+    """robertson.py:86-109: log-uniform rate scales and mass-conserving ICs."""
+    divergence = _check_divergence(divergence)
+    if divergence == 0.0:
+        params = np.broadcast_to(ROBERTSON_PARAMS, (size, 3)).copy()
+    else:
+        rng = np.random.default_rng(seed)
+        scales = 10.0 ** rng.uniform(-divergence, divergence, size=(size, 3))
+        params = ROBERTSON_PARAMS * scales
+    return robertson_initial_conditions(size, seed, divergence=divergence), params
+
+
+def robertson_lognormal_ensemble(size, seed=1, sigma=0.3):
+    """Stiff Robertson benchmark ensemble: k1 = 0.04 e^u1, k2 = 3e7 e^u2,
+    k3 = 1e4 e^u3 with u ~ N(0, sigma^2); y0 = (1, 0, 0); returned in the
+    reference's (k1, k3, k2) parameter order."""
+    rng = np.random.default_rng(seed)
+    u = rng.normal(0.0, sigma, size=(size, 3))
+    k1 = 0.04 * np.exp(u[:, 0])
+    k2 = 3e7 * np.exp(u[:, 1])
+    k3 = 1e4 * np.exp(u[:, 2])
+    y0 = np.tile(np.array([1.0, 0.0, 0.0]), (size, 1))
+    return y0, np.column_stack([k1, k3, k2])
+
+
+def robertson_benchmark_times():
+    """[0] and 49 log-spaced save times from 1e-5 to 1e5 (50 saves)."""
+    return np.concatenate([[0.0], np.logspace(-5.0, 5.0, 49)])
+
+
+# --------------------------------------------------------------------------
+# Van der Pol lattice (vdp.py), ring or open chain
+# --------------------------------------------------------------------------
+VDP_TIMES = np.array((0.0, 0.25, 0.5, 0.75, 1.0))
+
+
+def vdp_neighbours(n_osc, osc, coupling_range=1, wrap=True):
+    """vdp.py:235-250; an open chain drops the neighbours past either end."""
+    if coupling_range < 1:
+        raise ValueError("coupling_range must be at least 1")
+    found = []
+    for offset in range(1, coupling_range + 1):
+        for other in (osc - offset, osc + offset):
+            if wrap:
+                other %= n_osc
+            elif not 0 <= other < n_osc:
+                continue
+            if other != osc:
+                found.append(other)
+    return found
+
+
+def make_tuple_callback(name, components):
+    """``def name(y, t, p): return ((c0) + 0.0, ...)`` from expression strings,
+    the form the reference's reference systems are written in
+    (reference/systems/python/_tuple_codegen.py:17-30)."""
+    body = ",\n        ".join(f"({expr}) + 0.0" for expr in components)
+    source = f"def {name}(y, t, p):\n    return (\n        {body},\n    )\n"
+    namespace: dict = {}
+    exec(compile(source, f"<generated {name}>", "exec"), namespace)  # noqa: S102
+    fn = namespace[name]
+    fn._generated_source = source
+    return fn
+
+
+def vdp_lattice(n_osc, *, mu=100.0, d=10.0, omega=1.0, coupling_range=1, wrap=True):
+    """vdp.py:253-290; ``wrap=False`` gives the open chain."""
+    y0 = np.array([2.0, 0.0] * n_osc)
+    strength = d / coupling_range
+    components = []
+    for osc in range(n_osc):
+        base = 2 * osc
+        x, v = f"y[{base}]", f"y[{base + 1}]"
+        others = vdp_neighbours(n_osc, osc, coupling_range, wrap)
+        laplacian = " + ".join(
+            [f"y[{2 * other}]" for other in others] + [f"-{float(len(others))!r} * {x}"]
+        )
+        components += [
+            v,
+            f"p[0] * {mu!r} * (1.0 - {x} * {x}) * {v}"
+            f" + -{omega * omega!r} * {x}"
+            f" + {strength!r} * ({laplacian})",
+        ]
+    return make_tuple_callback("ode_fn", components), y0
+
+
+def vdp_sparsity(n_osc, coupling_range=1, wrap=True):
+    """vdp.py:293-307."""
+    n_vars = 2 * n_osc
+    mask = np.zeros((n_vars, n_vars), dtype=bool)
+    for osc in range(n_osc):
+        x, v = 2 * osc, 2 * osc + 1
+        mask[x, v] = True
+        mask[v, x] = mask[v, v] = True
+        for other in vdp_neighbours(n_osc, osc, coupling_range, wrap):
+            mask[v, 2 * other] = True
+    return mask
+
+
+def vdp_params(size, seed=42):
+    rng = np.random.default_rng(seed)
+    return np.array(1.0 + 0.2 * (2.0 * rng.random((size, 1)) - 1.0))
+
+
+def vdp_scenario(n_osc, size, seed=42, *, divergence=1.0):
+    """vdp.py:325-359."""
+    n_vars = 2 * n_osc
+    divergence = _check_divergence(divergence)
+    rng = np.random.default_rng(seed)
+    amplitudes = rng.uniform(0.25, 3.0, size=(size, n_osc))
+    signs = rng.choice(np.array([-1.0, 1.0]), size=(size, n_osc))
+    target_x = amplitudes * signs
+    position_divergence = min(divergence, 1.0)
+    x = 2.0 + position_divergence * (target_x - 2.0)
+    v = rng.normal(0.0, 2.0 * divergence, size=(size, n_osc))
+    y0 = np.empty((size, n_vars))
+    y0[:, 0::2] = x
+    y0[:, 1::2] = v
+    base_params = vdp_params(size, seed)
+    param_center = max(divergence, 1.0)
+    params = np.maximum(param_center + divergence * (base_params - 1.0), 1e-6)
+    return y0, params
+
+
+def vdp_chain_ensemble(n_osc, size, seed=3, mu_range=(10.0, 1000.0)):
+    """Stiff VdP chain benchmark ensemble: mu ~ LogUniform[mu_range] carried in
+    p[0] (the lattice is built with mu=1.0, coupling 0.1); y0 ~ U[-2, 2]."""
+    rng = np.random.default_rng(seed)
+    mu = np.exp(rng.uniform(np.log(mu_range[0]), np.log(mu_range[1]), size=(size, 1)))
+    y0 = rng.uniform(-2.0, 2.0, size=(size, 2 * n_osc))
+    return y0, mu
+
+
+# --------------------------------------------------------------------------
+# Small test callbacks from the reference's tests
+# --------------------------------------------------------------------------
+def forced_decay(y, t, p):
+    """tests/test_solver_options.py:142-143."""
+    return (-p[0] * y[0] + p[1] * t,)
+
+
+def forced_decay_exact(t, lam, forcing):
+    return forcing * (lam * t - 1.0 + np.exp(-lam * t)) / lam**2
+
+
+def decay(y, t, p):
+    return (-p[0] * y[0],)
+
+
+def chain4(y, t, p):
+    """tests/test_sparsity.py:40-46."""
+    return (
+        -p[0] * y[0],
+        p[0] * y[0] - p[1] * y[1],
+        p[1] * y[1] - p[2] * y[2],
+        p[2] * y[2],
+    )
+
+
+CHAIN4_MASK = np.zeros((4, 4), dtype=bool)
+for _r, _c in [(0, 0), (1, 0), (1, 1), (2, 1), (2, 2), (3, 2)]:
+    CHAIN4_MASK[_r, _c] = True
+
+
+def chain4_exact(t, a=0.7, b=0.4, c=0.2):
+    """tests/test_sparsity.py:49-61."""
+    y0 = np.exp(-a * t)
+    y1 = a / (b - a) * (np.exp(-a * t) - np.exp(-b * t))
+    y2 = a * b * (
+        np.exp(-a * t) / ((b - a) * (c - a))
+        + np.exp(-b * t) / ((a - b) * (c - b))
+        + np.exp(-c * t) / ((a - c) * (b - c))
+    )
+    return np.array([y0, y1, y2, 1.0 - y0 - y1 - y2])
+
+
+def ring6(y, t, p):
+    """tests/test_sparse_direct.py:333-342: six-point periodic diffusion with decay."""
+    return (
+        p[0] * (y[1] - 2.0 * y[0] + y[5]) - p[1] * y[0],
+        p[0] * (y[2] - 2.0 * y[1] + y[0]) - p[1] * y[1],
+        p[0] * (y[3] - 2.0 * y[2] + y[1]) - p[1] * y[2],
+        p[0] * (y[4] - 2.0 * y[3] + y[2]) - p[1] * y[3],
+        p[0] * (y[5] - 2.0 * y[4] + y[3]) - p[1] * y[4],
+        p[0] * (y[0] - 2.0 * y[5] + y[4]) - p[1] * y[5],
+    )
+
+
+def ring_mask(n):
+    mask = np.zeros((n, n), dtype=bool)
+    for i in range(n):
+        mask[i, i] = mask[i, (i + 1) % n] = mask[i, (i - 1) % n] = True
+    return mask
+
+
+def damped_oscillator(y, t, p):
+    """tests/test_save_hook.py:23-24."""
+    return (y[1], -p[0] * y[0] - p[1] * y[1])
+
+
+def kaps(n_pairs, epsilon_min):
+    """kaps.py:26-48; exact solution y1 = exp(-2t), y2 = exp(-t) for p[0] = 1."""
+    epsilon = [10.0 ** (np.log10(epsilon_min) * i / (n_pairs - 1)) for i in range(n_pairs)]
+    components = []
+    for pair, eps in enumerate(epsilon):
+        base = 2 * pair
+        inv_eps = 1.0 / float(eps)
+        y1, y2 = f"y[{base}]", f"y[{base + 1}]"
+        components += [
+            f"p[0] * (-{inv_eps + 2.0!r} * {y1} + {inv_eps!r} * ({y2} * {y2}))",
+            f"p[0] * (({y1} - {y2}) - {y2} * {y2})",
+        ]
+    fn = make_tuple_callback("ode_fn", components)
+    return fn, np.array([1.0, 1.0] * n_pairs)
+
+
+# --------------------------------------------------------------------------
+# The uneven stiff ensemble: dy/dt = A y + q (y*y), A block-arrow
+# --------------------------------------------------------------------------
+def arrow_structure(n=50, seed=7):
+    """Off-diagonal couplings of the block-arrow matrix and its declared mask.
+
+    Variables 0..n-3 pair into 2x2 blocks, variable n-2 stands alone and n-1 is
+    the hub. The coefficients that are actually non-zero keep A lower-arrow
+    triangular -- the upper entry of each 2x2 block and the hub row -- so A's
+    eigenvalues are its diagonal, which the parameters set per trajectory. The
+    declared sparsity mask is the full block arrow (both entries of every block,
+    the hub row and the hub column), a superset of the true pattern as the API
+    allows. Returns ``(rows, cols, vals, mask)``.
+    """
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    for b in range(0, n - 2, 2):
+        rows.append(b)
+        cols.append(b + 1)
+        vals.append(float(rng.uniform(0.5, 1.5)))
+    for j in range(n - 1):
+        rows.append(n - 1)
+        cols.append(j)
+        vals.append(float(rng.uniform(-0.2, 0.2)))
+    mask = np.eye(n, dtype=bool)
+    for b in range(0, n - 2, 2):
+        mask[b, b + 1] = mask[b + 1, b] = True
+    mask[n - 1, :] = True
+    mask[:, n - 1] = True
+    return np.array(rows), np.array(cols), np.array(vals), mask
+
+
+def arrow_system(n=50, quad=0.01, seed=7):
+    """``ode_fn`` for dy/dt = diag(p) y + offdiag(A) y + quad * y*y."""
+    rows, cols, vals, mask = arrow_structure(n, seed)
+    terms = [[f"p[{j}] * y[{j}]"] for j in range(n)]
+    for r, c, v in zip(rows, cols, vals):
+        terms[r].append(f"{float(v)!r} * y[{c}]")
+    components = [
+        " + ".join(t) + f" + {float(quad)!r} * (y[{j}] * y[{j}])" for j, t in enumerate(terms)
+    ]
+    fn = make_tuple_callback("arrow", components)
+    return fn, mask, (rows, cols, vals)
+
+
+def arrow_ensemble(size, n=50, seed=11, lam_range=(1e-1, 1e5), spread=(0.0, 4.0)):
+    """Per-trajectory eigenvalues for the uneven stiff ensemble.
+
+    Each trajectory draws a stiffness ceiling ``10**s`` with ``s`` uniform over
+    ``spread`` (decades above ``lam_range[0]``, capped at ``lam_range[1]``) and
+    then its ``n`` eigenvalues as -LogUniform[lam_range[0], ceiling]; y0 ~ U[-1, 1].
+    """
+    rng = np.random.default_rng(seed)
+    lo = math.log10(lam_range[0])
+    hi_cap = math.log10(lam_range[1])
+    s = rng.uniform(spread[0], spread[1], size=(size, 1))
+    ceiling = np.minimum(lo + 1.0 + s, hi_cap)
+    u = rng.uniform(0.0, 1.0, size=(size, n))
+    lam = -(10.0 ** (lo + u * (ceiling - lo)))
+    y0 = rng.uniform(-1.0, 1.0, size=(size, n))
+    return y0, lam
+
+
+def _check_divergence(divergence):
+    if not np.isfinite(divergence) or divergence < 0.0:
+        raise ValueError("divergence must be finite and non-negative")
+    return float(divergence)
