@@ -1,0 +1,107 @@
+/*
+ * menodax_hip.h -- C ABI of the MI355X ensemble ODE engine (libmenodax_hip.so).
+ *
+ * This is the drop-in boundary. The reference (lawrenceberry/menodax, package
+ * `modax`) reaches its GPU through three things in
+ * modax/_jax_numba_custom_call.py: a compile-and-load step that turns a numba
+ * kernel into a CUfunction (compile_kernel, :286-314), a launcher that takes
+ * that function, a stream, the launch shape and the solver's buffers
+ * (LaunchNumbaCudaAbi, :145-213), and the XLA FFI symbol that exposes the
+ * launcher (modax_numba_cuda_abi_launch, :215-231). The entry points below
+ * replace them one for one; the XLA handler built on them lives in
+ * menodax_amd/csrc/xla_ffi.cpp.
+ *
+ * Conventions: every function returns 0 on success and a nonzero code on
+ * failure, with the message available from mdx_last_error() on the calling
+ * thread. Kernel handles are int64 keys into a process-lifetime registry, like
+ * the reference's `function` attribute (:348-362). Nothing here allocates
+ * device memory on a launch, and nothing synchronises the stream.
+ */
+#ifndef MENODAX_HIP_H
+#define MENODAX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Version of this ABI; bumped on any signature change. */
+int32_t mdx_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* mdx_last_error(void);
+
+/* Compile HIP source to a gfx950 code object with hipRTC.
+ * Replaces the numba-cuda-mlir compile inside compile_kernel
+ * (modax/_jax_numba_custom_call.py:295-297). `headers`/`header_names` are
+ * in-memory include files; `options` are compiler options. On success
+ * *code and *log are malloc'd (free with mdx_free); on failure *log holds the
+ * compiler's diagnostics. */
+int32_t mdx_rtc_compile(const char* source, const char* name, int32_t n_headers,
+                        const char* const* headers, const char* const* header_names,
+                        int32_t n_options, const char* const* options, void** code,
+                        size_t* code_size, char** log);
+
+void mdx_free(void* p);
+
+/* Kernel flags for mdx_kernel_create. */
+#define MDX_KERNEL_PERSISTENT 1  /* refill kernel; module also holds mdx_reset_queue */
+#define MDX_KERNEL_UNIT_WEIGHTS 2 /* compiled with unit error weights: `weights` unread */
+
+/* Register a kernel from a code object: `entry` is its symbol, `block` the
+ * workgroup size it was compiled for, `flags` a combination of the
+ * MDX_KERNEL_* bits above. The code object is loaded on each device when first
+ * launched there; handles live as long as the process.
+ * Replaces cuModuleLoadData/cuModuleGetFunction in compile_kernel
+ * (modax/_jax_numba_custom_call.py:299-314). */
+int32_t mdx_kernel_create(const void* code, size_t code_size, const char* entry, int32_t block,
+                          int32_t flags, int64_t* handle);
+
+/* Resource use of a registered kernel on the current device: vector registers
+ * per lane, scratch (private) bytes per lane, static LDS bytes, and the
+ * workgroups per compute unit it can keep resident at its block size. No
+ * counterpart in the reference (numba's occupancy is not exposed there). */
+int32_t mdx_kernel_info(int64_t handle, int32_t* vgprs, int32_t* scratch_bytes,
+                        int32_t* lds_bytes, int32_t* blocks_per_cu);
+
+/* Workgroups a persistent launch should use on the current device: compute
+ * units times the resident workgroups per unit. */
+int32_t mdx_persistent_grid(int64_t handle, int32_t* grid);
+
+/* Device memory a launch of `handle` needs as `workspace` (the refill
+ * counter), in bytes. */
+int64_t mdx_workspace_bytes(int64_t handle, int64_t n_traj);
+
+/* Launch one ensemble solve on `stream` (a hipStream_t; NULL is the default
+ * stream). Replaces LaunchNumbaCudaAbi (modax/_jax_numba_custom_call.py:145-213)
+ * with the buffers in the reference kernel's argument order
+ * (modax/_numba_common.py:27-52, 84-130, rodas5P.py:680-694):
+ *   inputs  y0 (N, n_vars), times (n_save,), params (N, n_params),
+ *           scalars dt0, rtol, atol, max_steps, weights (N, n_vars);
+ *   outputs hist (N, n_save, n_vars), accepted, rejected, loop_steps (N,) int32,
+ *           hook_out (N, hook_size) or NULL.
+ * `layout` 0: every array trajectory-major and contiguous, as the reference
+ * passes them. `layout` 1: structure of arrays -- y0/weights (n_vars, N),
+ * params (n_params, N), hist (n_save, n_vars, N) -- the coalesced form.
+ * `grid` 0 sizes a one-lane-per-trajectory launch from N, or a persistent
+ * kernel's resident grid. `weights` may be NULL only for a kernel registered
+ * with MDX_KERNEL_UNIT_WEIGHTS. `workspace` must hold mdx_workspace_bytes; it
+ * is reset on the stream before the solve. */
+int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_traj,
+                         int32_t n_vars, int32_t n_params, int32_t n_save, int32_t layout,
+                         const double* y0, const double* times, const double* params, double dt0,
+                         double rtol, double atol, int32_t max_steps, const double* weights,
+                         double* hist, int32_t* accepted, int32_t* rejected,
+                         int32_t* loop_steps, double* hook_out, int32_t hook_size,
+                         void* workspace, int64_t workspace_bytes);
+
+/* Device facts used for code generation and the roofline: architecture name
+ * (e.g. "gfx950") and compute-unit count of the current device. */
+int32_t mdx_device_info(char* arch, size_t arch_cap, int32_t* n_cu);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MENODAX_HIP_H */

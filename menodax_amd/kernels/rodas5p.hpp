@@ -1,0 +1,431 @@
+// rodas5p.hpp -- Rodas5P ensemble kernels for gfx950, one trajectory per lane.
+//
+// Included by the source generated for one right-hand side, after it has
+// defined namespace sys { rhs, neg_jac, [sparse_factor, sparse_solve, DIAG] }
+// and these macros:
+//   MDX_N, MDX_NP        state and parameter widths
+//   MDX_NLU              elements of the iteration-matrix buffer
+//   MDX_LU_FP32          1: iteration matrix, its LU and the stage vectors in
+//                        float; 0: double
+//   MDX_SPARSE           1: pattern-compiled sparse LU from sys; 0: dense LU
+//   MDX_NEEDS_CLEAR      1 when some buffer slot is written by no sweep
+//   MDX_N_ERROR          components the error norm divides by
+//   MDX_TF_INDEX         parameter column holding the end time, or -1
+//   MDX_BLOCK            lanes per workgroup
+//   MDX_PARAMS_IN_REGS   keep the parameter row in registers
+//   MDX_E1/E2/E3         PID exponents
+//
+// The method, stage by stage, is the reference kernel's (modax/rodas5P.py:
+// tableau 44-105, controller 108-113, 340-342, loop 761-989, tail 991-1005);
+// the precision of each value follows the reference's numba typing (stage
+// right-hand sides accumulated in double and rounded to the LU type, stage
+// vectors stored in the LU type, state/error/controller in double).
+#pragma once
+#include "mdx_common.hpp"
+
+namespace mdx {
+namespace r5 {
+
+#if MDX_LU_FP32
+typedef float LUT;
+#else
+typedef double LUT;
+#endif
+
+constexpr int N = MDX_N;
+constexpr double GAMMA = 0.21193756319429014;
+
+// W-transformed Rodas5P coefficients (Steinebach 2023), rows padded.
+__device__ constexpr double CM[8][7] = {
+    {0, 0, 0, 0, 0, 0, 0},
+    {-14.155112264123755, 0, 0, 0, 0, 0, 0},
+    {-17.97296035885952, -2.859693295451294, 0, 0, 0, 0, 0},
+    {147.12150275711716, -1.41221402718213, 71.68940251302358, 0, 0, 0, 0},
+    {165.43517024871676, -0.4592823456491126, 42.90938336958603, -5.961986721573306, 0, 0, 0},
+    {24.854864614690072, -3.0009227002832186, 47.4931110020768, 5.5814197821558125,
+     -0.6610691825249471, 0, 0},
+    {30.91273214028599, -3.1208243349937974, 77.79954646070892, 34.28646028294783,
+     -19.097331116725623, -28.087943162872662, 0},
+    {37.80277123390563, -3.2571969029072276, 112.26918849496327, 66.9347231244047,
+     -40.06618937091002, -54.66780262877968, -9.48861652309627},
+};
+__device__ constexpr double DV[8] = {0.21193756319429014, -0.42387512638858027,
+                                     -0.3384627126235924, 1.8046452872882734,
+                                     2.325825639765069,   0.0, 0.0, 0.0};
+__device__ constexpr double AM[5][5] = {
+    {3.0, 0, 0, 0, 0},
+    {2.849394379747939, 0.45842242204463923, 0, 0, 0},
+    {-6.954028509809101, 2.489845061869568, -10.358996098473584, 0, 0},
+    {2.8029986275628964, 0.5072464736228206, -0.3988312541770524, -0.04721187230404641, 0},
+    {-7.502846399306121, 2.561846144803919, -11.627539656261098, -0.18268767659942256,
+     0.030198172008377946},
+};
+// Stage abscissae c2..c5 (c6 = c7 = c8 = 1 use t_end).
+__device__ constexpr double CT[4] = {0.6358126895828704, 0.4095798393397535,
+                                     0.9769306725060716, 0.4288403609558664};
+// Dense-output coefficients (rodas5P.py:939-968).
+__device__ constexpr double H1[8] = {25.948786856663858, -2.5579724845846235, 10.433815404888879,
+                                     -2.3679251022685204, 0.524948541321073, 1.1241088310450404,
+                                     0.4272876194431874, -0.17202221070155493};
+__device__ constexpr double H2[8] = {-9.91568850695171, -0.9689944594115154, 3.0438037242978453,
+                                     -24.495224566215796, 20.176138334709044, 15.98066361424651,
+                                     -6.789040303419874, -6.710236069923372};
+__device__ constexpr double H3[8] = {11.419903575922262, 2.8879645146136994, 72.92137995996029,
+                                     80.12511834622643, -52.072871366152654, -59.78993625266729,
+                                     -0.15582684282751913, 4.883087185713722};
+
+#if MDX_PARAMS_IN_REGS
+typedef RegParams<MDX_NP> PA;
+#else
+typedef GlobalParams PA;
+#endif
+
+// ---------------------------------------------------------------------------
+// Dense LU with partial pivoting, in registers (rodas5P.py:187-236). The pivot
+// row is chosen first max wins, the whole row is swapped, and a zero pivot
+// leaves its column alone. The swap is a select against every candidate row so
+// all indices stay compile-time constants and nothing leaves the register file.
+// In float storage the reciprocal and multiplier are double and each update is
+// one double fma rounded to float, as the reference's typing has it.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void dense_factor(T* __restrict__ lu, int* __restrict__ ipiv) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    T maxv = absT(lu[i * N + i]);
+    int piv = i;
+#pragma unroll
+    for (int k = i + 1; k < N; ++k) {
+      const T v = absT(lu[k * N + i]);
+      if (v > maxv) {
+        maxv = v;
+        piv = k;
+      }
+    }
+    ipiv[i] = piv;
+#pragma unroll
+    for (int k = i + 1; k < N; ++k) {
+      const bool sw = (piv == k);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const T a = lu[i * N + j], b = lu[k * N + j];
+        lu[i * N + j] = sw ? b : a;
+        lu[k * N + j] = sw ? a : b;
+      }
+    }
+    const T pv = lu[i * N + i];
+    if (pv != (T)0) {
+      const double inv = 1.0 / (double)pv;
+#pragma unroll
+      for (int k = i + 1; k < N; ++k) {
+        const double f = (double)lu[k * N + i] * inv;
+        lu[k * N + i] = (T)f;
+#pragma unroll
+        for (int j = i + 1; j < N; ++j)
+          lu[k * N + j] = (T)fmaT(-f, (double)lu[i * N + j], (double)lu[k * N + j]);
+      }
+    }
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void dense_solve(const T* __restrict__ lu, const int* __restrict__ ipiv,
+                                            T* __restrict__ r) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int piv = ipiv[i];
+#pragma unroll
+    for (int k = i + 1; k < N; ++k) {
+      const bool sw = (piv == k);
+      const T a = r[i], b = r[k];
+      r[i] = sw ? b : a;
+      r[k] = sw ? a : b;
+    }
+    T acc = r[i];
+#pragma unroll
+    for (int j = 0; j < i; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
+    r[i] = acc;
+  }
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    T acc = r[i];
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
+    r[i] = acc / lu[i * N + i];
+  }
+}
+
+__device__ __forceinline__ int diag_slot(int d) {
+#if MDX_SPARSE
+  return sys::DIAG[d];
+#else
+  return d * N + d;
+#endif
+}
+
+__device__ __forceinline__ void factor(LUT* __restrict__ lu, int* __restrict__ ipiv) {
+#if MDX_SPARSE
+  (void)ipiv;
+  sys::sparse_factor(lu);
+#else
+  dense_factor(lu, ipiv);
+#endif
+}
+
+__device__ __forceinline__ void solve(const LUT* __restrict__ lu, const int* __restrict__ ipiv,
+                                      LUT* __restrict__ r) {
+#if MDX_SPARSE
+  (void)ipiv;
+  sys::sparse_solve(lu, r);
+#else
+  dense_solve(lu, ipiv, r);
+#endif
+}
+
+// Stage S: rhs_j = LU(work_j + dt*d_S*dT_j + (sum_q C_Sq k_qj) / dt), solved
+// against the step's factorisation into k_S (rodas5P.py:611-647).
+template <int S>
+__device__ __forceinline__ void stage(const double* __restrict__ work,
+                                      const double* __restrict__ dT, LUT (*__restrict__ k)[N],
+                                      const LUT* __restrict__ lu, const int* __restrict__ ipiv,
+                                      double dt_use, double inv_dt) {
+  LUT r[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double v;
+    if constexpr (S == 0) {
+      v = fmaT(dt_use * DV[0], dT[j], work[j]);
+    } else {
+      double acc = CM[S][0] * (double)k[0][j];
+#pragma unroll
+      for (int q = 1; q < S; ++q) acc = fmaT(CM[S][q], (double)k[q][j], acc);
+      if constexpr (S < 5)
+        v = fmaT(acc, inv_dt, fmaT(dt_use * DV[S], dT[j], work[j]));
+      else
+        v = fmaT(acc, inv_dt, work[j]);
+    }
+    r[j] = (LUT)v;
+  }
+  solve(lu, ipiv, r);
+#pragma unroll
+  for (int j = 0; j < N; ++j) k[S][j] = r[j];
+}
+
+// u = y + sum_q A_{S+1,q} k_q (rodas5P.py:649-661).
+template <int S>
+__device__ __forceinline__ void advance(double* __restrict__ u, const double* __restrict__ y,
+                                        LUT (*__restrict__ k)[N]) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double acc = AM[S][0] * (double)k[0][j];
+#pragma unroll
+    for (int q = 1; q <= S; ++q) acc = fmaT(AM[S][q], (double)k[q][j], acc);
+    u[j] = y[j] + acc;
+  }
+}
+
+// Everything one trajectory carries from step to step.
+struct State {
+  double y[N];
+  double t, dt, tf, err_prev, err_prev2;
+  int save_idx, n_steps, accepted, rejected;
+};
+
+__device__ __forceinline__ void init(State& s, const SolveArgs& a, i64 i, const PA& p) {
+  const double* __restrict__ times = a.times;
+  const double tf = times[a.n_save - 1];
+#pragma unroll
+  for (int j = 0; j < N; ++j) s.y[j] = a.y0[i * a.y0_ts + (i64)j * a.y0_vs];
+#pragma unroll
+  for (int j = 0; j < N; ++j) a.hist[i * a.h_ts + (i64)j * a.h_vs] = s.y[j];
+  s.t = times[0];
+  s.dt = a.dt0 > 0.0 ? a.dt0 : (tf - times[0]) * 1e-6;
+#if MDX_TF_INDEX >= 0
+  s.tf = p[MDX_TF_INDEX];
+#else
+  (void)p;
+  s.tf = tf;
+#endif
+  s.err_prev = 1.0;
+  s.err_prev2 = 1.0;
+  s.save_idx = 1;
+  s.n_steps = 0;
+  s.accepted = 0;
+  s.rejected = 0;
+}
+
+__device__ __forceinline__ bool active(const State& s, const SolveArgs& a) {
+  return s.save_idx < a.n_save && s.t < s.tf && s.n_steps < a.max_steps;
+}
+
+// One attempted step, accepted or rejected, with the saves it passes.
+__device__ __forceinline__ void step(State& s, const PA& p, const SolveArgs& a, i64 i) {
+  double dt_use = s.dt;
+  if (dt_use > s.tf - s.t) dt_use = s.tf - s.t;
+  if (dt_use < 1e-30) dt_use = 1e-30;
+  const double inv_dt = 1.0 / dt_use;
+  const double t = s.t;
+  const double t_end = t + dt_use;
+  const double dtgamma_inv = 1.0 / (dt_use * GAMMA);
+
+  // M = I/(h gamma) - J straight into the buffer the factorisation works in.
+  LUT lu[MDX_NLU];
+#if MDX_NEEDS_CLEAR
+#pragma unroll
+  for (int q = 0; q < MDX_NLU; ++q) lu[q] = (LUT)0;
+#endif
+  double dT[N];
+  sys::neg_jac(s.y, t, p, lu, dT);
+#pragma unroll
+  for (int d = 0; d < N; ++d) lu[diag_slot(d)] = lu[diag_slot(d)] + (LUT)dtgamma_inv;
+  int ipiv[N];
+  factor(lu, ipiv);
+
+  double work[N], u[N];
+  LUT k[8][N];
+  sys::rhs(s.y, t, p, work);
+  stage<0>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  advance<0>(u, s.y, k);
+  sys::rhs(u, t + CT[0] * dt_use, p, work);
+  stage<1>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  advance<1>(u, s.y, k);
+  sys::rhs(u, t + CT[1] * dt_use, p, work);
+  stage<2>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  advance<2>(u, s.y, k);
+  sys::rhs(u, t + CT[2] * dt_use, p, work);
+  stage<3>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  advance<3>(u, s.y, k);
+  sys::rhs(u, t + CT[3] * dt_use, p, work);
+  stage<4>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  advance<4>(u, s.y, k);
+  sys::rhs(u, t_end, p, work);
+  stage<5>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+#pragma unroll
+  for (int j = 0; j < N; ++j) u[j] = u[j] + (double)k[5][j];
+  sys::rhs(u, t_end, p, work);
+  stage<6>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+#pragma unroll
+  for (int j = 0; j < N; ++j) u[j] = u[j] + (double)k[6][j];
+  sys::rhs(u, t_end, p, work);
+  stage<7>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+
+  // Weighted RMS error of the embedded estimate k7 (rodas5P.py:903-911).
+  double err = 0.0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const double k7 = (double)k[7][j];
+    const double y_new = u[j] + k7;
+    const double ya = absT(s.y[j]), yb = absT(y_new);
+    const double scale = fmaT(a.rtol, ya > yb ? ya : yb, a.atol);
+    const double r = weight(a, i, j) * k7 / scale;
+    err = fmaT(r, r, err);
+  }
+  const double err_norm = __builtin_sqrt(err / (double)MDX_N_ERROR);
+  const bool accept = err_norm <= 1.0 && !__builtin_isnan(err_norm);
+  const double factor_ =
+      pid_factor(err_norm, accept, s.err_prev, s.err_prev2, MDX_E1, MDX_E2, MDX_E3, 0.2, 6.0);
+  s.dt = dt_use * factor_;
+
+  if (accept) {
+    const double t_new = t + dt_use;
+    const double* __restrict__ times = a.times;
+    while (s.save_idx < a.n_save) {
+      const double ts = times[s.save_idx];
+      const double at = absT(ts);
+      if (!(ts <= t_new + 1e-12 * (1.0 > at ? 1.0 : at))) break;
+      const double theta = (ts - t) / dt_use;
+      const double theta1 = 1.0 - theta;
+      double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        double h1 = H1[0] * (double)k[0][j];
+        double h2 = H2[0] * (double)k[0][j];
+        double h3 = H3[0] * (double)k[0][j];
+#pragma unroll
+        for (int q = 1; q < 8; ++q) {
+          const double kv = (double)k[q][j];
+          h1 = fmaT(H1[q], kv, h1);
+          h2 = fmaT(H2[q], kv, h2);
+          h3 = fmaT(H3[q], kv, h3);
+        }
+        const double y_new = u[j] + (double)k[7][j];
+        double v = fmaT(theta, h3, h2);
+        v = fmaT(theta, v, h1);
+        v = fmaT(theta1, v, y_new);
+        row[(i64)j * a.h_vs] = fmaT(theta, v, theta1 * s.y[j]);
+      }
+      s.save_idx++;
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) s.y[j] = u[j] + (double)k[7][j];
+    s.t = t + dt_use;
+    s.accepted++;
+  } else {
+    s.rejected++;
+  }
+  s.n_steps++;
+}
+
+// Saves past the trajectory's end hold its final state; counters out.
+__device__ __forceinline__ void finish(State& s, const SolveArgs& a, i64 i) {
+  while (s.save_idx < a.n_save) {
+    double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
+#pragma unroll
+    for (int j = 0; j < N; ++j) row[(i64)j * a.h_vs] = s.y[j];
+    s.save_idx++;
+  }
+  a.accepted[i] = s.accepted;
+  a.rejected[i] = s.rejected;
+  a.loop[i] = s.n_steps;
+}
+
+}  // namespace r5
+}  // namespace mdx
+
+// One lane per trajectory, every lane stepping until its own trajectory is done.
+extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_rodas5p(mdx::SolveArgs a) {
+  using namespace mdx::r5;
+  const mdx::i64 i = (mdx::i64)blockIdx.x * MDX_BLOCK + threadIdx.x;
+  if (i >= a.n_traj) return;
+  PA p;
+  p.load(a, i);
+  State s;
+  init(s, a, i, p);
+  while (active(s, a)) step(s, p, a, i);
+  finish(s, a, i);
+}
+
+// Persistent refill: a resident grid whose lanes claim trajectories from a
+// device counter and take the next one as soon as theirs ends, so a wave keeps
+// every lane busy while step counts across the ensemble differ by orders of
+// magnitude. Per trajectory it runs the same init/step/finish as mdx_rodas5p,
+// hence the same bits. Every lane leaves once the counter has passed n_traj
+// and its last trajectory is finished.
+extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_rodas5p_refill(mdx::SolveArgs a) {
+  using namespace mdx::r5;
+  PA p;
+  State s;
+  mdx::i64 i = -1;
+  bool exhausted = false;
+  for (;;) {
+    const bool want = (i < 0) && !exhausted;
+    const mdx::i64 got = mdx::wave_claim(a.queue, want);
+    if (want) {
+      if (got < a.n_traj) {
+        i = got;
+        p.load(a, i);
+        init(s, a, i, p);
+      } else {
+        exhausted = true;
+      }
+    }
+    if (!__any(i >= 0)) break;
+    if (i >= 0) {
+      if (active(s, a)) step(s, p, a, i);
+      if (!active(s, a)) {
+        finish(s, a, i);
+        i = -1;
+      }
+    }
+  }
+}

@@ -311,7 +311,7 @@ static void sys_jac(const osys* s, const double* y, double t, const double* p, d
         double ie = s->inv_eps[pr];
         int a = 2 * pr, b = a + 1;
         J[a * n + a] = p[0] * (-(ie + 2.0));
-        J[a * n + b] = p[0] * ie * 2.0 * y[b];
+        J[a * n + b] = p[0] * (ie * (2.0 * y[b]));
         J[b * n + a] = p[0];
         J[b * n + b] = p[0] * (-1.0 - 2.0 * y[b]);
       }
