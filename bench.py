@@ -1,0 +1,291 @@
+#!/usr/bin/env python
+"""Ensemble Rodas5P throughput on one MI355X.
+
+Workload (the metric's configuration): the stiff Robertson ensemble, 65,536
+trajectories, rates k1 = 0.04 e^u1, k2 = 3e7 e^u2, k3 = 1e4 e^u3 with
+u ~ N(0, 0.3^2) per trajectory (seed 1), y0 = (1, 0, 0), t in [0, 1e5] with 50
+saves ([0] and 49 log-spaced from 1e-5), rtol 1e-6, atol 1e-8, dense LU,
+double-precision state. A "step" of this benchmark is one whole ensemble solve
+with inputs resident in HBM; it is timed for the FP32-LU kernel (the headline
+``value``) and for the FP64-LU kernel beside it.
+
+The JSON line also carries
+  - ``roofline``: the solve kernel's algorithmic FP64/FP32 flops (counted from
+    the generated code and the kernel's own step counters) over its average
+    duration measured with HIP events on the launch stream, as FP64-equivalent
+    TFLOP/s against the FP64 vector peak -- the kernels are VALU bound, with no
+    shared contraction for the matrix cores and HBM touched once per input
+    and save;
+  - ``cpu_baseline``: the C oracle (OpenMP over trajectories, the same
+    algorithm and inputs) on the host's cores;
+  - ``endpoint_error``: the error of both LU precisions against a tight
+    (rtol 1e-12, FP64) oracle solve on a sample of trajectories;
+  - ``reference_config``: the reference's own published benchmark inputs
+    (100,000 identical Robertson trajectories, benchmarks/stiff_robertson_scaling)
+    for comparison with its RTX 4070 SUPER numbers.
+
+``python bench.py --gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_TRAJ = 65536
+RTOL, ATOL = 1e-6, 1e-8
+REF_RTX4070_FP32_TRAJ_PER_S = 100000 / 35.97e-3  # stiff_robertson_scaling csv:12
+REF_RTX4070_FP64_TRAJ_PER_S = 100000 / 45.59e-3  # csv:23
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_solves(run, steps, warmup, torch):
+    """Warm up, then time exactly ``steps`` solves bracketed by synchronisation;
+    per-launch HIP events on the launch stream give the kernel's own time."""
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    stream = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        starts[k].record(stream)
+        out = run()
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    event_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    return out, wall, float(np.mean(event_ms))
+
+
+def endpoint_error(hist_gpu, y0, params, times, sample, oracle, O, lu):
+    """max and p99 over the sample of max_j |y_gpu - y_true| / (atol + rtol |y_true|)
+    at the final save, y_true from the FP64 oracle at rtol 1e-12."""
+    idx = np.arange(sample)
+    truth, _ = O.rodas5p(oracle, y0[idx], times, params[idx], rtol=1e-12, atol=1e-14,
+                         lu_precision="fp64")
+    yt = truth[:, -1, :]
+    yg = hist_gpu[idx, -1, :]
+    e = np.max(np.abs(yg - yt) / (ATOL + RTOL * np.abs(yt)), axis=1)
+    return {"lu_precision": lu, "sample": int(sample), "max": float(e.max()),
+            "p99": float(np.quantile(e, 0.99)), "max_abs": float(np.max(np.abs(yg - yt))),
+            "units": "multiples of atol + rtol*|y_true| at t_end"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=N_TRAJ)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="budget for the CPU oracle baseline sample")
+    ap.add_argument("--refill", choices=("auto", "on", "off"), default="auto")
+    ap.add_argument("--skip-reference-config", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(
+            "bench.py measures one GPU: trajectories are independent, so a multi-GPU run "
+            "is N/G trajectories per GPU with no exchange (see DESIGN.md); run with --gpus 1",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    import torch
+
+    from menodax_amd import _kernels, _runtime, roofline, rodas5P, systems as S
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    arch, n_cu = _runtime.device_info()
+    log(f"bench: device {arch}, {n_cu} CUs, N={args.n}")
+
+    y0_np, p_np = S.robertson_lognormal_ensemble(args.n, seed=1)
+    times_np = S.robertson_benchmark_times()
+    # inputs resident in HBM in the structure-of-arrays layout; the (N, k)
+    # views handed to the launcher are transposes of contiguous storage, so the
+    # launch makes no copies
+    y0_s = torch.tensor(np.ascontiguousarray(y0_np.T), device=dev)
+    p_s = torch.tensor(np.ascontiguousarray(p_np.T), device=dev)
+    times = torch.tensor(times_np, device=dev)
+    y0_v, p_v = y0_s.t(), p_s.t()
+    refill = {"auto": None, "on": True, "off": False}[args.refill]
+
+    results = {}
+    for lu in ("fp32", "fp64"):
+        built = _kernels.rodas5p_kernel(S.robertson, 3, 3, _kernels.KernelOptions(lu_precision=lu))
+        info = _runtime.kernel_info(built.plain)
+        use_refill = rodas5P._use_refill(built, args.n) if refill is None else refill
+        outputs = rodas5P.launch(built, y0_v, times, p_v, args.n, dt0=0.0, rtol=RTOL, atol=ATOL,
+                                 max_steps=100000, refill=use_refill)
+        ws = outputs._keep[-1]
+
+        def run(built=built, outputs=outputs, ws=ws, use_refill=use_refill):
+            return rodas5P.launch(built, y0_v, times, p_v, args.n, dt0=0.0, rtol=RTOL, atol=ATOL,
+                                  max_steps=100000, refill=use_refill, outputs=outputs,
+                                  workspace=ws)
+
+        log(f"bench: timing lu={lu} refill={use_refill} vgprs={info['vgprs']} "
+            f"scratch={info['scratch_bytes']}B blocks/CU={info['blocks_per_cu']}")
+        out, wall, event_ms = time_solves(run, args.steps, args.warmup, torch)
+        acc = out.accepted.cpu().numpy().astype(np.int64)
+        rej = out.rejected.cpu().numpy().astype(np.int64)
+        hist = out.ensemble_view().cpu().numpy()
+        attempted = int((acc + rej).sum())
+        saves = args.n * (times_np.size - 1)
+        work = roofline.solve_work(built, attempted, saves)
+        t_kernel = event_ms / 1e3
+        results[lu] = dict(
+            built=built, info=info, refill=use_refill, wall=wall, event_ms=event_ms,
+            ms_per_step=wall / args.steps * 1e3, accepted=int(acc.sum()), rejected=int(rej.sum()),
+            attempted=attempted, work=work, t_kernel=t_kernel, hist=hist,
+            max_attempted=int((acc + rej).max()), min_attempted=int((acc + rej).min()),
+        )
+        log(f"bench: lu={lu} {results[lu]['ms_per_step']:.3f} ms/solve (events {event_ms:.3f} ms), "
+            f"{args.n / (results[lu]['ms_per_step'] / 1e3):.4g} traj/s, "
+            f"mean attempted {attempted / args.n:.2f}")
+
+    r32, r64 = results["fp32"], results["fp64"]
+    traj_s = args.n / (r32["ms_per_step"] / 1e3)
+    w = r32["work"]
+    roof_t = w.roof_seconds()
+    fp64_equiv = w.fp64 + w.fp32 * roofline.P64_TFLOPS / roofline.P32_TFLOPS
+    achieved = fp64_equiv / r32["t_kernel"] / 1e12
+
+    line = {
+        "metric": "trajectories/s + accepted steps/s, Rodas5P FP32-LU vs FP64 at rtol 1e-6, 1 GPU",
+        "value": traj_s,
+        "unit": "trajectories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 state, f32 LU",
+        "data": "synthetic (seeded lognormal Robertson rates)",
+        "config": {
+            "workload": "Robertson Rodas5P dense LU, lognormal rates sigma 0.3, 50 log-spaced "
+                        "saves to t=1e5, rtol 1e-6 atol 1e-8",
+            "n_traj": args.n,
+            "n_vars": 3,
+            "n_save": int(times_np.size),
+            "lu_precision": "fp32",
+            "kernel": "refill" if r32["refill"] else "one lane per trajectory",
+        },
+        "accepted_steps_per_s": r32["accepted"] / (r32["ms_per_step"] / 1e3),
+        "fp64_lu": {
+            "trajectories_per_s": args.n / (r64["ms_per_step"] / 1e3),
+            "accepted_steps_per_s": r64["accepted"] / (r64["ms_per_step"] / 1e3),
+            "ms_per_step": r64["ms_per_step"],
+        },
+        "fp32_over_fp64": r64["ms_per_step"] / r32["ms_per_step"],
+        "steps_per_trajectory": {
+            "fp32": {"attempted_mean": r32["attempted"] / args.n, "min": r32["min_attempted"],
+                     "max": r32["max_attempted"], "rejected_mean": r32["rejected"] / args.n},
+            "fp64": {"attempted_mean": r64["attempted"] / args.n, "min": r64["min_attempted"],
+                     "max": r64["max_attempted"], "rejected_mean": r64["rejected"] / args.n},
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": roofline.P64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / roofline.P64_TFLOPS,
+            "traffic": None,
+            "kernel": "mdx_rodas5p_refill" if r32["refill"] else "mdx_rodas5p",
+            "kernel_ms": r32["t_kernel"] * 1e3,
+            "flops_fp64": w.fp64,
+            "flops_fp32": w.fp32,
+            "special_fn_calls": w.special,
+            "roof_ms": roof_t * 1e3,
+            "note": "achieved = (fp64 flops + fp32 flops * 78.6/157.3) / kernel time: "
+                    "FP64-equivalent TFLOP/s against the FP64 vector peak; frac = roof time "
+                    "/ kernel time",
+            "hbm_algorithmic_bytes": roofline.hbm_bytes(args.n, 3, 3, int(times_np.size), False),
+            "vgprs": r32["info"]["vgprs"],
+            "scratch_bytes": r32["info"]["scratch_bytes"],
+        },
+    }
+
+    # error against a tight FP64 oracle on a sample
+    from oracle import oracle as O
+
+    sample = min(args.n, 512)
+    line["endpoint_error"] = [
+        endpoint_error(results[lu]["hist"], y0_np, p_np, times_np, sample, O.robertson(), O, lu)
+        for lu in ("fp32", "fp64")
+    ]
+
+    if not args.no_cpu_baseline:
+        threads = O.max_threads()
+        # a bounded sample of the same workload: time growing slices until the
+        # budget is spent, report the rate of the last one
+        n_cpu = 1024
+        while True:
+            t0 = time.perf_counter()
+            O.rodas5p(O.robertson(), y0_np[:n_cpu], times_np, p_np[:n_cpu], rtol=RTOL, atol=ATOL,
+                      lu_precision="fp32", threads=threads)
+            dt = time.perf_counter() - t0
+            if dt > args.cpu_seconds / 3 or n_cpu >= args.n:
+                break
+            n_cpu = min(args.n, n_cpu * 4)
+        line["cpu_baseline"] = {
+            "value": n_cpu / dt,
+            "unit": "trajectories/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"first {n_cpu} trajectories of the same ensemble, FP32-LU, C oracle "
+                      f"with OpenMP, {dt:.2f} s",
+        }
+
+    if not args.skip_reference_config:
+        # the reference's own published inputs (stiff_robertson_scaling, N=100000)
+        n_ref = 100000
+        y0r = torch.tensor(np.ascontiguousarray(np.tile(S.ROBERTSON_Y0, (n_ref, 1)).T), device=dev)
+        pr = torch.tensor(np.ascontiguousarray(np.tile(S.ROBERTSON_PARAMS, (n_ref, 1)).T), device=dev)
+        tr = torch.tensor(S.ROBERTSON_TIMES, device=dev)
+        ref = {}
+        for lu in ("fp32", "fp64"):
+            built = _kernels.rodas5p_kernel(S.robertson, 3, 3, _kernels.KernelOptions(lu_precision=lu))
+            outs = rodas5P.launch(built, y0r.t(), tr, pr.t(), n_ref, dt0=1e-4, rtol=RTOL, atol=ATOL,
+                                  max_steps=100000, refill=False)
+
+            def run_ref(built=built, outs=outs):
+                return rodas5P.launch(built, y0r.t(), tr, pr.t(), n_ref, dt0=1e-4, rtol=RTOL,
+                                      atol=ATOL, max_steps=100000, refill=False, outputs=outs)
+
+            o, wall, _ = time_solves(run_ref, 5, 1, torch)
+            acc = int(o.accepted.sum().item())
+            ms = wall / 5 * 1e3
+            ref[lu] = {"ms": ms, "trajectories_per_s": n_ref / (ms / 1e3),
+                       "accepted_steps_per_s": acc / (ms / 1e3),
+                       "attempted_per_traj": float((o.accepted + o.rejected).float().mean().item())}
+        ref["fp32"]["vs_rtx4070_super"] = ref["fp32"]["trajectories_per_s"] / REF_RTX4070_FP32_TRAJ_PER_S
+        ref["fp64"]["vs_rtx4070_super"] = ref["fp64"]["trajectories_per_s"] / REF_RTX4070_FP64_TRAJ_PER_S
+        line["reference_config"] = {
+            "workload": "100,000 identical Robertson trajectories, y0=(0.891,0.1,0.009), "
+                        "p=(0.04,1e4,3e7), 5 saves to 1e5, first_step 1e-4, rtol 1e-6, atol 1e-8",
+            "published_rtx4070_super_ms": {"fp32": 35.97, "fp64": 45.59},
+            **ref,
+        }
+
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -113,11 +113,18 @@ def headers():
     return out
 
 
+#: code objects compiled ahead of time by ``build`` travel with the tree here
+PACKAGE_CACHE = os.path.join(_HERE, "_kcache")
+
+
 def _cache_dirs():
+    """Lookup order for compiled code objects; the first writable one is where
+    a fresh compile is stored."""
     dirs = []
     env = os.environ.get("MENODAX_KERNEL_CACHE")
     if env:
         dirs.append(env)
+    dirs.append(PACKAGE_CACHE)
     dirs.append(os.path.join(os.path.expanduser("~"), ".cache", "menodax_amd", "kernels"))
     return dirs
 
@@ -163,11 +170,21 @@ def rtc_compile(source: str, name: str = "mdx_kernel.hip", hdrs=None, options=CO
 _memory_cache: dict[str, bytes] = {}
 
 
-def code_object(source: str, name: str = "mdx_kernel.hip") -> bytes:
+def code_object(source: str, name: str = "mdx_kernel.hip", store_in: str | None = None) -> bytes:
     """Compile through the cache: memory, then disk (keyed on source, headers
-    and options), then hipRTC."""
+    and options), then hipRTC. ``store_in`` forces a copy into that directory."""
     hdrs = headers()
     key = cache_key(source, hdrs, COMPILE_OPTIONS)
+    if store_in is not None:
+        data = code_object(source, name)
+        os.makedirs(store_in, exist_ok=True)
+        path = os.path.join(store_in, key + ".co")
+        if not os.path.exists(path):
+            tmp = f"{path}.{os.getpid()}.tmp"
+            with open(tmp, "wb") as f:
+                f.write(data)
+            os.replace(tmp, path)
+        return data
     hit = _memory_cache.get(key)
     if hit is not None:
         return hit

@@ -88,41 +88,87 @@ typedef GlobalParams PA;
 // In float storage the reciprocal and multiplier are double and each update is
 // one double fma rounded to float, as the reference's typing has it.
 // ---------------------------------------------------------------------------
+// Up to this size the dense LU is unrolled with every index a constant, so the
+// matrix stays in registers; above it the loops are kept and the matrix lives
+// in scratch (a dense solve at that size is the uninformative end of the
+// sparse path, which is where larger systems belong).
+#ifndef MDX_DENSE_UNROLL_MAX
+#define MDX_DENSE_UNROLL_MAX 16
+#endif
+constexpr bool DENSE_UNROLLED = N <= MDX_DENSE_UNROLL_MAX;
+
 template <class T>
 __device__ __forceinline__ void dense_factor(T* __restrict__ lu, int* __restrict__ ipiv) {
+  if constexpr (DENSE_UNROLLED) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    T maxv = absT(lu[i * N + i]);
-    int piv = i;
-#pragma unroll
-    for (int k = i + 1; k < N; ++k) {
-      const T v = absT(lu[k * N + i]);
-      if (v > maxv) {
-        maxv = v;
-        piv = k;
-      }
-    }
-    ipiv[i] = piv;
-#pragma unroll
-    for (int k = i + 1; k < N; ++k) {
-      const bool sw = (piv == k);
-#pragma unroll
-      for (int j = 0; j < N; ++j) {
-        const T a = lu[i * N + j], b = lu[k * N + j];
-        lu[i * N + j] = sw ? b : a;
-        lu[k * N + j] = sw ? a : b;
-      }
-    }
-    const T pv = lu[i * N + i];
-    if (pv != (T)0) {
-      const double inv = 1.0 / (double)pv;
+    for (int i = 0; i < N; ++i) {
+      T maxv = absT(lu[i * N + i]);
+      int piv = i;
 #pragma unroll
       for (int k = i + 1; k < N; ++k) {
-        const double f = (double)lu[k * N + i] * inv;
-        lu[k * N + i] = (T)f;
+        const T v = absT(lu[k * N + i]);
+        if (v > maxv) {
+          maxv = v;
+          piv = k;
+        }
+      }
+      ipiv[i] = piv;
 #pragma unroll
-        for (int j = i + 1; j < N; ++j)
-          lu[k * N + j] = (T)fmaT(-f, (double)lu[i * N + j], (double)lu[k * N + j]);
+      for (int k = i + 1; k < N; ++k) {
+        const bool sw = (piv == k);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          const T a = lu[i * N + j], b = lu[k * N + j];
+          lu[i * N + j] = sw ? b : a;
+          lu[k * N + j] = sw ? a : b;
+        }
+      }
+      const T pv = lu[i * N + i];
+      if (pv != (T)0) {
+        const double inv = 1.0 / (double)pv;
+#pragma unroll
+        for (int k = i + 1; k < N; ++k) {
+          const double f = (double)lu[k * N + i] * inv;
+          lu[k * N + i] = (T)f;
+#pragma unroll
+          for (int j = i + 1; j < N; ++j)
+            lu[k * N + j] = (T)fmaT(-f, (double)lu[i * N + j], (double)lu[k * N + j]);
+        }
+      }
+    }
+  } else {
+#pragma unroll 1
+    for (int i = 0; i < N; ++i) {
+      T maxv = absT(lu[i * N + i]);
+      int piv = i;
+#pragma unroll 1
+      for (int k = i + 1; k < N; ++k) {
+        const T v = absT(lu[k * N + i]);
+        if (v > maxv) {
+          maxv = v;
+          piv = k;
+        }
+      }
+      ipiv[i] = piv;
+      if (piv != i) {
+#pragma unroll 1
+        for (int j = 0; j < N; ++j) {
+          const T a = lu[i * N + j];
+          lu[i * N + j] = lu[piv * N + j];
+          lu[piv * N + j] = a;
+        }
+      }
+      const T pv = lu[i * N + i];
+      if (pv != (T)0) {
+        const double inv = 1.0 / (double)pv;
+#pragma unroll 1
+        for (int k = i + 1; k < N; ++k) {
+          const double f = (double)lu[k * N + i] * inv;
+          lu[k * N + i] = (T)f;
+#pragma unroll 4
+          for (int j = i + 1; j < N; ++j)
+            lu[k * N + j] = (T)fmaT(-f, (double)lu[i * N + j], (double)lu[k * N + j]);
+        }
       }
     }
   }
@@ -131,27 +177,50 @@ __device__ __forceinline__ void dense_factor(T* __restrict__ lu, int* __restrict
 template <class T>
 __device__ __forceinline__ void dense_solve(const T* __restrict__ lu, const int* __restrict__ ipiv,
                                             T* __restrict__ r) {
+  if constexpr (DENSE_UNROLLED) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const int piv = ipiv[i];
+    for (int i = 0; i < N; ++i) {
+      const int piv = ipiv[i];
 #pragma unroll
-    for (int k = i + 1; k < N; ++k) {
-      const bool sw = (piv == k);
-      const T a = r[i], b = r[k];
-      r[i] = sw ? b : a;
-      r[k] = sw ? a : b;
+      for (int k = i + 1; k < N; ++k) {
+        const bool sw = (piv == k);
+        const T a = r[i], b = r[k];
+        r[i] = sw ? b : a;
+        r[k] = sw ? a : b;
+      }
+      T acc = r[i];
+#pragma unroll
+      for (int j = 0; j < i; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
+      r[i] = acc;
     }
-    T acc = r[i];
 #pragma unroll
-    for (int j = 0; j < i; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
-    r[i] = acc;
-  }
+    for (int i = N - 1; i >= 0; --i) {
+      T acc = r[i];
 #pragma unroll
-  for (int i = N - 1; i >= 0; --i) {
-    T acc = r[i];
-#pragma unroll
-    for (int j = i + 1; j < N; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
-    r[i] = acc / lu[i * N + i];
+      for (int j = i + 1; j < N; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
+      r[i] = acc / lu[i * N + i];
+    }
+  } else {
+#pragma unroll 1
+    for (int i = 0; i < N; ++i) {
+      const int piv = ipiv[i];
+      if (piv != i) {
+        const T a = r[i];
+        r[i] = r[piv];
+        r[piv] = a;
+      }
+      T acc = r[i];
+#pragma unroll 4
+      for (int j = 0; j < i; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
+      r[i] = acc;
+    }
+#pragma unroll 1
+    for (int i = N - 1; i >= 0; --i) {
+      T acc = r[i];
+#pragma unroll 4
+      for (int j = i + 1; j < N; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
+      r[i] = acc / lu[i * N + i];
+    }
   }
 }
 

@@ -29,9 +29,10 @@
  *
  * Rounding. The reference runs on a GPU whose compiler contracts a multiply
  * feeding an add into one fused multiply-add; its published Robertson step
- * counts (157 attempted, 10 rejected with the FP32 LU) are only reproduced when
- * the f32 triangular solves are fused that way. So this file spells out every
- * fusion as an explicit fma() and is built with -ffp-contract=off: the product's
+ * counts (157 attempted, 10 rejected with the FP32 LU) are not reproduced
+ * without fusing the f32 triangular solves that way (and are reproduced exactly
+ * with the fusions below). So this file spells out every fusion as an explicit
+ * fma() and is built with -ffp-contract=off: the product's
  * HIP kernels place the same fma()s in the same spots under the same flag, so
  * the two agree bit for bit wherever the right-hand side is polynomial. The
  * step-size controller's power is the deterministic det_pow() below (built from

@@ -11,9 +11,10 @@ independently, which is what makes a parity test between them mean something.
 
 Parity status of each oracle function (how it is pinned, see the tests named):
   - ``rodas5p`` dense, LU fp32: reference goldens -- Robertson 157 attempted /
-    10 rejected (benchmarks/stiff_robertson_divergence csv:2) and the seeded
-    divergence distributions (csv:3-12); closed forms (forced decay, chain,
-    decay, Kaps).  tests/test_oracle.py
+    10 rejected exactly (benchmarks/stiff_robertson_divergence csv:2), the
+    seeded divergence distributions (csv:3-12) to 2e-5 in the mean, the 96-D
+    stiff VdP ring's 21 steps (stiff_vdp_divergence csv:2); closed forms
+    (forced decay, chain, Kaps). tests/test_oracle.py
   - ``rodas5p`` dense, LU fp64: closed forms at tight tolerance; convergence order.
   - ``rodas5p`` sparse: against its own dense path on the same system (the
     reference's dense-vs-sparse tests, tests/test_sparse_direct.py:347-382)

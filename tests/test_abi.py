@@ -1,0 +1,101 @@
+"""The C ABI of libmenodax_hip.so, without a GPU.
+
+Loads the library, checks it exports every function ``include/menodax_hip.h``
+declares, and exercises the calls that need no device: compilation through
+hipRTC, kernel registration, and the launcher's argument validation, which
+rejects malformed launches before touching the GPU (the reference's handler
+does the same with kInvalidArgument, modax/_jax_numba_custom_call.py:137-196).
+"""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from menodax_amd import _kernels, _runtime
+from menodax_amd import systems as S
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "menodax_hip.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int32_t|int64_t|void|const char\*)\s+(mdx_\w+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_function():
+    lib = _runtime.lib()
+    names = _declared()
+    assert len(names) >= 9
+    for name in names:
+        assert hasattr(lib, name), name
+    assert lib.mdx_abi_version() == 1
+
+
+def test_unknown_handle_is_reported():
+    with pytest.raises(_runtime.NativeError, match="unknown kernel handle"):
+        _runtime.solve_launch(999999, None, 0, 4, 3, 3, 5, 1, 1, 1, 1, 0.0, 1e-6, 1e-8, 10, 0,
+                              1, 1, 1, 1)
+
+
+@pytest.fixture(scope="module")
+def handles():
+    src, _ = _kernels.rodas5p_source(S.robertson, 3, 3, _kernels.KernelOptions(unit_weights=False))
+    code = _runtime.rtc_compile(src, "abi.hip")
+    plain = _runtime.kernel_create(code, "mdx_rodas5p", 64, 0)
+    refill = _runtime.kernel_create(code, "mdx_rodas5p_refill", 64, _runtime.KERNEL_PERSISTENT)
+    return plain, refill
+
+
+def _launch(handle, **over):
+    kw = dict(grid=0, n_traj=4, n_vars=3, n_params=3, n_save=5, layout=1, y0=8, times=8, params=8,
+              dt0=0.0, rtol=1e-6, atol=1e-8, max_steps=10, weights=8, hist=8, accepted=8,
+              rejected=8, loop_steps=8, hook_out=0, hook_size=0, workspace=0, workspace_bytes=0)
+    kw.update(over)
+    _runtime.solve_launch(handle, None, **kw)
+
+
+@pytest.mark.parametrize(
+    "override, message",
+    [
+        (dict(n_traj=-1), "bad extents"),
+        (dict(n_vars=0), "bad extents"),
+        (dict(n_save=0), "bad extents"),
+        (dict(layout=7), "layout must be 0 or 1"),
+        (dict(hist=0), "null buffer"),
+        (dict(weights=0), "reads error weights"),
+        (dict(max_steps=-3), "max_steps"),
+        (dict(hook_size=2), "needs hook_out"),
+    ],
+)
+def test_malformed_launches_are_rejected_before_the_gpu(handles, override, message):
+    with pytest.raises(_runtime.NativeError, match=message):
+        _launch(handles[0], **override)
+
+
+def test_a_persistent_kernel_needs_its_workspace(handles):
+    with pytest.raises(_runtime.NativeError, match="workspace"):
+        _launch(handles[1])
+
+
+def test_an_empty_ensemble_launches_nothing(handles):
+    _launch(handles[0], n_traj=0)  # returns before any device call
+
+
+def test_kernel_create_validates():
+    with pytest.raises(_runtime.NativeError, match="block must be"):
+        _runtime.kernel_create(b"\x7fELF....", "x", 0, 0)
+
+
+def test_compile_returns_a_gfx950_code_object():
+    src, _ = _kernels.tsit5_source(S.lorenz, 3, 1, _kernels.Tsit5Options())
+    code = _runtime.rtc_compile(src, "lorenz.hip")
+    assert code[:4] == b"\x7fELF"
+    assert b"gfx950" in code
+
+
+def test_last_error_is_thread_local_text():
+    lib = _runtime.lib()
+    assert isinstance(lib.mdx_last_error(), bytes)
+    assert ctypes.c_int32(lib.mdx_abi_version()).value == 1

@@ -1,0 +1,183 @@
+"""The tracer, the forward-mode derivatives and the emitted HIP source.
+
+The derivatives replace the reference's Enzyme sweeps
+(modax/_sensitivity.py:148-179 via rodas5P.py:445-500), which its tests hold
+against JAX's jacfwd (tests/test_enzyme_jacobian.py:109-138). Here they are held
+against the oracle's hand-written closed-form Jacobians, bit for bit, through a
+host interpreter that rounds exactly as the emitted code does; and against
+central differences for functions the oracle does not carry.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from menodax_amd import _codegen as cg
+from menodax_amd import _kernels, _runtime
+from menodax_amd import systems as S
+from oracle import oracle as O
+
+CASES = [
+    ("robertson", S.robertson, 3, 3, O.robertson),
+    ("lorenz", S.lorenz, 3, 1, O.lorenz),
+    ("forced_decay", S.forced_decay, 1, 2, O.forced_decay),
+    ("chain4", S.chain4, 4, 3, O.chain4),
+    ("ring6", S.ring6, 6, 2, O.ring6),
+    ("damped", S.damped_oscillator, 2, 2, O.damped_osc),
+    ("vdp_chain", S.vdp_lattice(5, mu=1.0, d=0.1, wrap=False)[0], 10, 1,
+     lambda: O.vdp_lattice(5, mu=1.0, d=0.1, wrap=False)),
+    ("vdp_ring", S.vdp_lattice(6)[0], 12, 1, lambda: O.vdp_lattice(6)),
+    ("kaps", S.kaps(3, 1e-3)[0], 6, 1, lambda: O.kaps(3, 1e-3)),
+]
+
+
+@pytest.mark.parametrize("name, fn, n, n_p, osys", CASES, ids=[c[0] for c in CASES])
+def test_rhs_and_jacobian_are_bit_exact_with_the_closed_forms(name, fn, n, n_p, osys):
+    traced = cg.trace(fn, n, n_p)
+    system = osys()
+    rng = np.random.default_rng(0)
+    for _ in range(25):
+        y = rng.normal(size=n) * 3
+        p = rng.uniform(0.1, 5, size=n_p)
+        t = rng.uniform(0, 2)
+        assert np.array_equal(cg.rhs_from_graph(traced, y, t, p), O.rhs(system, y, t, p))
+        J, dT = cg.jacobian_from_graph(traced, y, t, p)
+        Jo, dTo = O.jacobian(system, y, t, p)
+        assert np.array_equal(J, Jo)
+        assert np.array_equal(dT, dTo)
+
+
+def test_block_arrow_jacobian_bit_exact():
+    fn, mask, (rows, cols, vals) = S.arrow_system(12)
+    traced = cg.trace(fn, 12, 12)
+    system = O.linquad(12, rows, cols, vals, 0.01)
+    rng = np.random.default_rng(1)
+    y, p = rng.normal(size=12), -rng.uniform(0.1, 100, size=12)
+    J, _ = cg.jacobian_from_graph(traced, y, 0.0, p)
+    assert np.array_equal(J, O.jacobian(system, y, 0.0, p)[0])
+
+
+def test_coloured_sweeps_recover_the_jacobian():
+    """Seeding a whole colour group per sweep returns every column of the group."""
+    from menodax_amd._sparse_direct import sparse_direct_solver_for
+    from menodax_amd._sparsity import normalize_sparsity
+
+    fn, _ = S.vdp_lattice(8, mu=3.0, d=0.1, wrap=False)
+    mask = S.vdp_sparsity(8, wrap=False)
+    compressed = sparse_direct_solver_for(normalize_sparsity(mask, 16), "amd").compressed
+    assert compressed.n_colours < 16
+    traced = cg.trace(fn, 16, 1)
+    rng = np.random.default_rng(2)
+    y, p = rng.normal(size=16), np.array([7.0])
+    full, _ = cg.jacobian_from_graph(traced, y, 0.0, p)
+    coloured, _ = cg.jacobian_from_graph(traced, y, 0.0, p, colour=compressed.colour,
+                                         n_colours=compressed.n_colours)
+    assert np.array_equal(full, coloured)
+
+
+def _fd_jacobian(fn, y, t, p, h=1e-6):
+    n = len(y)
+    J = np.zeros((n, n))
+    for c in range(n):
+        e = np.zeros(n)
+        e[c] = h
+        J[:, c] = (np.array(fn(y + e, t, p)) - np.array(fn(y - e, t, p))) / (2 * h)
+    dT = (np.array(fn(y, t + h, p)) - np.array(fn(y, t - h, p))) / (2 * h)
+    return J, dT
+
+
+def test_transcendental_callbacks_against_central_differences():
+    def f(y, t, p):
+        return (
+            math.exp(-p[0] * y[0]) * math.sin(y[1] * t) + abs(y[2]) ** 1.5,
+            math.log(1.0 + y[0] * y[0]) / math.sqrt(2.0 + y[1] ** 2) - max(y[0], y[2]),
+            math.tanh(y[2]) * math.cos(p[1] * t) + min(y[0] * 3.0, 1.0) + y[1] ** -2,
+        )
+
+    traced = cg.trace(f, 3, 2)
+    rng = np.random.default_rng(3)
+    for _ in range(10):
+        y = rng.uniform(0.3, 1.5, size=3) * np.array([1, 1, -1])
+        p = rng.uniform(0.5, 2, size=2)
+        t = rng.uniform(0.1, 2)
+        np.testing.assert_allclose(cg.rhs_from_graph(traced, y, t, p), f(y, t, p), rtol=1e-14)
+        J, dT = cg.jacobian_from_graph(traced, y, t, p)
+        Jf, dTf = _fd_jacobian(f, y, t, p)
+        np.testing.assert_allclose(J, Jf, rtol=1e-6, atol=1e-7)
+        np.testing.assert_allclose(dT, dTf, rtol=1e-6, atol=1e-7)
+
+
+def test_integer_powers_square_like_numba():
+    """x**3 is x*(x*x) and x**4 is (x*x)*(x*x): numba's exponentiation by squaring."""
+    g = cg.Graph()
+    x = g.y(0)
+    n3 = cg._int_power(g, x, 3)
+    assert n3.op == "mul" and {n3.args[0].id, n3.args[1].id} == {x.id, g.mul(x, x).id}
+    n4 = cg._int_power(g, x, 4)
+    sq = g.mul(x, x)
+    assert n4.op == "mul" and n4.args[0] is sq and n4.args[1] is sq
+
+
+def test_where_selects_and_branching_raises():
+    def ok(y, t, p):
+        return (cg.where(y[0] > 0.0, y[0], -y[0]),)
+
+    tr = cg.trace(ok, 1, 0)
+    assert cg.rhs_from_graph(tr, np.array([-2.0]), 0.0, np.array([]))[0] == 2.0
+
+    def bad(y, t, p):
+        return (y[0] if y[0] > 0 else -y[0],)
+
+    with pytest.raises(cg.TraceError, match="branches"):
+        cg.trace(bad, 1, 0)
+
+
+def test_wrong_component_count_is_an_error():
+    with pytest.raises(ValueError, match="returned 2 components"):
+        cg.trace(lambda y, t, p: (y[0], y[1]), 3, 0)
+
+
+def test_math_module_is_restored_after_tracing():
+    cg.trace(lambda y, t, p: (math.exp(y[0]),), 1, 0)
+    assert math.exp(0.0) == 1.0 and math.exp.__module__ == "math"
+
+
+def test_constants_are_emitted_exactly():
+    for v in (0.1, 1.0 / 3.0, -2.5e-300, 6.02214076e23, -0.0):
+        assert float.fromhex(cg._lit(v)) == v if v != -0.0 else cg._lit(v) == "-0.0"
+
+
+def test_flop_counts_of_robertson():
+    traced = cg.trace(S.robertson, 3, 3)
+    _, fc = cg.generate_rhs(traced)
+    # -p0*y0 + (p1*y1)*y2 ; p0*y0 - ... - p2*(y1*y1) ; p2*(y1*y1), shared products counted once
+    assert 8 <= fc.flops <= 12 and fc.transcendentals == 0
+
+
+# --- the generated kernels compile for gfx950 (hipRTC needs no GPU) ----------
+@pytest.mark.parametrize("lu", ["fp32", "fp64"])
+def test_rodas5p_dense_kernel_compiles(lu):
+    src, _ = _kernels.rodas5p_source(S.robertson, 3, 3, _kernels.KernelOptions(lu_precision=lu))
+    code = _runtime.rtc_compile(src, "t.hip")
+    assert code[:4] == b"\x7fELF"
+
+
+def test_rodas5p_sparse_kernel_compiles():
+    from menodax_amd._sparsity import normalize_sparsity
+
+    fn, mask, _ = S.arrow_system(50)
+    opts = _kernels.KernelOptions(sparsity=normalize_sparsity(mask, 50), unit_weights=False)
+    src, d = _kernels.rodas5p_source(fn, 50, 50, opts)
+    assert "sparse_factor" in src and d["structure"].size >= int(mask.sum())
+    assert _runtime.rtc_compile(src, "arrow.hip")[:4] == b"\x7fELF"
+
+
+def test_tsit5_kernel_compiles():
+    src, _ = _kernels.tsit5_source(S.lorenz, 3, 1, _kernels.Tsit5Options(unit_weights=False))
+    assert _runtime.rtc_compile(src, "lorenz.hip")[:4] == b"\x7fELF"
+
+
+def test_a_compile_error_reports_the_source():
+    with pytest.raises(_runtime.NativeError, match="hipRTC compilation"):
+        _runtime.rtc_compile("this is not C++", "bad.hip")

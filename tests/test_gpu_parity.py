@@ -1,0 +1,302 @@
+"""Parity of the HIP kernels with the oracle, on an MI355X.
+
+The kernels and the oracle place every fused multiply-add in the same spot, use
+the same deterministic power in the controller, and the right-hand sides here
+are polynomial, so the bar is bit-exact: equal step counters for every
+trajectory and identical histories (``np.array_equal``), in both LU precisions
+and both launch shapes. Published reference goldens are checked on the GPU too.
+"""
+
+import numpy as np
+import pytest
+
+from menodax_amd import rodas5P, systems as S, tsit5
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(hist, stats, ref_hist, ref_stats):
+    for k in ref_stats:
+        np.testing.assert_array_equal(np.asarray(stats[k]), ref_stats[k], err_msg=k)
+    np.testing.assert_array_equal(hist, ref_hist)
+
+
+# --- Rodas5P, dense -----------------------------------------------------------
+@pytest.mark.parametrize("lu", ["fp32", "fp64"])
+@pytest.mark.parametrize("refill", [False, True])
+def test_robertson_ensemble_bit_exact(lu, refill):
+    y0, p = S.robertson_lognormal_ensemble(2000, seed=1)
+    t = S.robertson_benchmark_times()
+    hist, st = rodas5P.solve(S.robertson, y0, t, p, rtol=1e-6, atol=1e-8, lu_precision=lu,
+                             return_stats=True, refill=refill)
+    ref, rst = O.rodas5p(O.robertson(), y0, t, p, rtol=1e-6, atol=1e-8, lu_precision=lu)
+    _same(hist, st, ref, rst)
+
+
+def test_robertson_published_single_trajectory():
+    """benchmarks/stiff_robertson_divergence csv:2: 157 attempted, 10 rejected
+    (initial state built as robertson.py:8-10 builds it)."""
+    hist, st = rodas5P.solve(S.robertson, S.ROBERTSON_Y0[None], S.ROBERTSON_TIMES,
+                             S.ROBERTSON_PARAMS[None], first_step=1e-4, rtol=1e-6, atol=1e-8,
+                             lu_precision="fp32", return_stats=True)
+    assert {k: int(v[0]) for k, v in st.items()} == dict(
+        accepted_steps=147, rejected_steps=10, loop_steps=157)
+    np.testing.assert_allclose(hist[0].sum(axis=-1), 1.0, rtol=1e-6)  # mass conservation
+
+
+def test_robertson_divergence_distribution_on_gpu():
+    y0, p = S.robertson_scenario(30000, 42, divergence=1.0)
+    _, st = rodas5P.solve(S.robertson, y0, S.ROBERTSON_TIMES, p, first_step=1e-4, rtol=1e-6,
+                          atol=1e-8, lu_precision="fp32", return_stats=True)
+    _, rst = O.rodas5p(O.robertson(), y0, S.ROBERTSON_TIMES, p, first_step=1e-4, rtol=1e-6,
+                       atol=1e-8, lu_precision="fp32")
+    for k in rst:
+        np.testing.assert_array_equal(st[k], rst[k])
+    att = st["accepted_steps"] + st["rejected_steps"]
+    assert att.mean() == pytest.approx(106.2629, rel=2e-4)  # published csv:6
+    assert (att.min(), att.max()) == (32, 184)
+
+
+@pytest.mark.parametrize("lu", ["fp32", "fp64"])
+@pytest.mark.parametrize("n_osc", [8, 12])
+def test_stiff_vdp_ring_dense_bit_exact(lu, n_osc):
+    """Stiff VdP ring (mu = 100), dense LU with pivoting: 16 variables take the
+    register-resident unrolled LU, 24 the looped one."""
+    fn, _ = S.vdp_lattice(n_osc)
+    y0, p = S.vdp_scenario(n_osc, 70, 42, divergence=0.25)
+    hist, st = rodas5P.solve(fn, y0, S.VDP_TIMES, p, first_step=1e-4, rtol=1e-6, atol=1e-8,
+                             lu_precision=lu, return_stats=True)
+    ref, rst = O.rodas5p(O.vdp_lattice(n_osc), y0, S.VDP_TIMES, p, first_step=1e-4, rtol=1e-6,
+                         atol=1e-8, lu_precision=lu)
+    _same(hist, st, ref, rst)
+
+
+def test_forced_decay_closed_form_on_gpu():
+    """tests/test_solver_options.py:159-176."""
+    t = np.linspace(0.0, 1.0, 11)
+    out = rodas5P.solve(S.forced_decay, np.zeros((1, 1)), t, np.array([[10.0, 5.0]]), rtol=1e-10,
+                        atol=1e-12, first_step=1e-4, lu_precision="fp64")
+    np.testing.assert_allclose(out[0, :, 0], S.forced_decay_exact(t, 10.0, 5.0), rtol=1e-8,
+                               atol=1e-10)
+
+
+def test_default_first_step_matches_explicit():
+    """tests/test_solvers.py:71-96: first_step=None is 1e-6 of the window."""
+    fn, _ = S.vdp_lattice(1, mu=1.0)
+    y0, p = S.vdp_scenario(1, 8, 42, divergence=1.0)
+    window = float(S.VDP_TIMES[-1] - S.VDP_TIMES[0])
+    for solve in (rodas5P.solve, tsit5.solve):
+        a = solve(fn, y0, S.VDP_TIMES, p, rtol=1e-6, atol=1e-8, first_step=window * 1e-6)
+        b = solve(fn, y0, S.VDP_TIMES, p, rtol=1e-6, atol=1e-8)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_tf_index_ends_each_trajectory_at_its_own_time():
+    """tests/test_solver_options.py:179-206."""
+    t = np.linspace(0.0, 1.0, 11)
+    params = np.array([[10.0, 5.0, 1.0], [10.0, 5.0, 0.5]])
+    out = rodas5P.solve(S.forced_decay, np.zeros((2, 1)), t, params, tf_index=2, rtol=1e-10,
+                        atol=1e-12, first_step=1e-4, lu_precision="fp64")
+    exact = S.forced_decay_exact(t, 10.0, 5.0)
+    np.testing.assert_allclose(out[0, :, 0], exact, rtol=1e-8, atol=1e-10)
+    stop = int(np.searchsorted(t, 0.5, side="right"))
+    np.testing.assert_allclose(out[1, :stop, 0], exact[:stop], rtol=1e-8, atol=1e-10)
+    assert (out[1, stop:, 0] == out[1, stop - 1, 0]).all()
+
+
+def test_max_steps_exhaustion_holds_the_last_state():
+    y0, p = S.robertson_lognormal_ensemble(100, seed=4)
+    t = S.robertson_benchmark_times()
+    hist, st = rodas5P.solve(S.robertson, y0, t, p, rtol=1e-6, atol=1e-8, max_steps=7,
+                             return_stats=True)
+    ref, rst = O.rodas5p(O.robertson(), y0, t, p, rtol=1e-6, atol=1e-8, max_steps=7)
+    _same(hist, st, ref, rst)
+    assert (st["loop_steps"] == 7).all()
+
+
+def test_error_weights_and_pid_gains_bit_exact():
+    y0, p = S.robertson_lognormal_ensemble(300, seed=5)
+    t = S.robertson_benchmark_times()
+    w = np.random.default_rng(0).uniform(0.0, 2.0, size=(300, 3))
+    kw = dict(rtol=1e-7, atol=1e-9, error_weights=w, pcoeff=0.2, icoeff=0.6, dcoeff=0.1)
+    hist, st = rodas5P.solve(S.robertson, y0, t, p, return_stats=True, lu_precision="fp64", **kw)
+    ref, rst = O.rodas5p(O.robertson(), y0, t, p, lu_precision="fp64", weights=w,
+                         **{k: v for k, v in kw.items() if k != "error_weights"})
+    _same(hist, st, ref, rst)
+
+
+@pytest.mark.parametrize("n", [1, 63, 65, 129])
+def test_partial_waves_and_broadcast_inputs(n):
+    """1-D y0 broadcast over a 2-D params (modax/_jax_common.py:12-46); partial
+    last wavefronts."""
+    _, p = S.robertson_lognormal_ensemble(n, seed=6)
+    y0 = np.array([1.0, 0.0, 0.0])
+    t = S.robertson_benchmark_times()
+    hist, st = rodas5P.solve(S.robertson, y0, t, p, rtol=1e-6, atol=1e-8, return_stats=True)
+    ref, rst = O.rodas5p(O.robertson(), np.tile(y0, (n, 1)), t, p, rtol=1e-6, atol=1e-8)
+    assert hist.shape == (n, t.size, 3)
+    _same(hist, st, ref, rst)
+
+
+def test_a_single_save_time_takes_no_step():
+    y0, p = S.robertson_lognormal_ensemble(10, seed=7)
+    hist, st = rodas5P.solve(S.robertson, y0, np.array([0.0]), p, return_stats=True)
+    assert (st["loop_steps"] == 0).all()
+    np.testing.assert_array_equal(hist[:, 0, :], y0)
+
+
+def test_torch_inputs_give_torch_outputs_and_both_layouts_agree():
+    import torch
+
+    from menodax_amd import _common, _kernels, _runtime
+
+    y0, p = S.robertson_lognormal_ensemble(500, seed=8)
+    t = S.robertson_benchmark_times()
+    dev = torch.device("cuda", 0)
+    out = rodas5P.solve(S.robertson, torch.tensor(y0, device=dev), torch.tensor(t, device=dev),
+                        torch.tensor(p, device=dev), rtol=1e-6, atol=1e-8)
+    assert isinstance(out, torch.Tensor) and out.shape == (500, t.size, 3)
+    # the trajectory-major layout through the C ABI directly
+    built = _kernels.rodas5p_kernel(S.robertson, 3, 3, _kernels.KernelOptions())
+    y0d = torch.tensor(y0, device=dev)
+    pd = torch.tensor(p, device=dev)
+    td = torch.tensor(t, device=dev)
+    hist = torch.empty((500, t.size, 3), dtype=torch.float64, device=dev)
+    acc, rej, lp = (torch.empty(500, dtype=torch.int32, device=dev) for _ in range(3))
+    _runtime.solve_launch(built.plain, torch.cuda.current_stream().cuda_stream, 0, 500, 3, 3,
+                          t.size, _common.LAYOUT_TRAJECTORY_MAJOR, y0d.data_ptr(), td.data_ptr(),
+                          pd.data_ptr(), 0.0, 1e-6, 1e-8, 100000, 0, hist.data_ptr(),
+                          acc.data_ptr(), rej.data_ptr(), lp.data_ptr())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(hist.cpu().numpy(), out.cpu().numpy())
+
+
+# --- Rodas5P, sparse ------------------------------------------------------------
+@pytest.mark.parametrize("lu", ["fp32", "fp64"])
+@pytest.mark.parametrize("ordering", ["amd", "natural"])
+def test_ring_with_fill_in_sparse_bit_exact(lu, ordering):
+    """tests/test_sparse_direct.py:367-382: a pattern whose LU fills in, so the
+    buffer must be cleared before the sweeps."""
+    from menodax_amd import _sparse_direct as SD
+    from menodax_amd._sparsity import normalize_sparsity
+
+    mask = S.ring_mask(6)
+    y0 = np.zeros((64, 6))
+    y0[:, 0] = 1.0
+    p = np.tile(np.array([50.0, 1.0]), (64, 1))
+    p[:, 0] *= np.linspace(0.5, 1.5, 64)
+    t = np.array([0.0, 0.01, 0.2])
+    kw = dict(rtol=1e-9, atol=1e-11, first_step=1e-6, lu_precision=lu)
+    hist, st = rodas5P.solve(S.ring6, y0, t, p, sparsity=mask, ordering=ordering,
+                             return_stats=True, **kw)
+    layout = SD.oracle_tables(normalize_sparsity(mask, 6), ordering)
+    ref, rst = O.rodas5p(O.ring6(), y0, t, p, layout=layout, **kw)
+    _same(hist, st, ref, rst)
+    dense = rodas5P.solve(S.ring6, y0, t, p, **kw)
+    np.testing.assert_allclose(hist, dense, rtol=1e-6 if lu == "fp32" else 1e-8, atol=1e-10)
+
+
+def test_chain_sparse_matches_closed_form():
+    """tests/test_sparsity.py:113-127."""
+    y0 = np.tile(np.array([1.0, 0.0, 0.0, 0.0]), (64, 1))
+    p = np.tile(np.array([0.7, 0.4, 0.2]), (64, 1))
+    t = np.array([0.0, 0.5, 2.0])
+    kw = dict(rtol=1e-11, atol=1e-13, first_step=1e-4, lu_precision="fp64")
+    dense = rodas5P.solve(S.chain4, y0, t, p, **kw)
+    sparse = rodas5P.solve(S.chain4, y0, t, p, sparsity=S.CHAIN4_MASK, **kw)
+    np.testing.assert_allclose(sparse, dense, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(sparse[0, -1], S.chain4_exact(2.0), rtol=1e-9, atol=1e-11)
+
+
+@pytest.mark.parametrize("refill", [False, True])
+def test_vdp_chain_sparse_fp32_bit_exact(refill):
+    """The stiff VdP chain (64 variables, block-tridiagonal), sparse FP32 LU with AMD."""
+    from menodax_amd import _sparse_direct as SD
+    from menodax_amd._sparsity import normalize_sparsity
+
+    fn, _ = S.vdp_lattice(32, mu=1.0, d=0.1, wrap=False)
+    mask = S.vdp_sparsity(32, wrap=False)
+    y0, mu = S.vdp_chain_ensemble(32, 256)
+    t = np.linspace(0.0, 200.0, 11)
+    hist, st = rodas5P.solve(fn, y0, t, mu, sparsity=mask, rtol=1e-6, atol=1e-8,
+                             lu_precision="fp32", return_stats=True, refill=refill)
+    layout = SD.oracle_tables(normalize_sparsity(mask, 64), "amd")
+    ref, rst = O.rodas5p(O.vdp_lattice(32, mu=1.0, d=0.1, wrap=False), y0, t, mu, layout=layout,
+                         rtol=1e-6, atol=1e-8, lu_precision="fp32")
+    _same(hist, st, ref, rst)
+
+
+def test_block_arrow_sparse_bit_exact():
+    """The uneven stiff ensemble's system, 50 variables, sparse FP32 LU."""
+    from menodax_amd import _sparse_direct as SD
+    from menodax_amd._sparsity import normalize_sparsity
+
+    fn, mask, (rows, cols, vals) = S.arrow_system(50)
+    y0, lam = S.arrow_ensemble(300)
+    t = np.linspace(0.0, 10.0, 20)
+    for lu in ("fp32", "fp64"):
+        hist, st = rodas5P.solve(fn, y0, t, lam, sparsity=mask, rtol=1e-6, atol=1e-8,
+                                 lu_precision=lu, return_stats=True, refill=True)
+        layout = SD.oracle_tables(normalize_sparsity(mask, 50), "amd")
+        ref, rst = O.rodas5p(O.linquad(50, rows, cols, vals, 0.01), y0, t, lam, layout=layout,
+                             rtol=1e-6, atol=1e-8, lu_precision=lu)
+        _same(hist, st, ref, rst)
+
+
+# --- Tsit5 --------------------------------------------------------------------
+@pytest.mark.parametrize("refill", [False, True])
+def test_lorenz_rho_ensemble_bit_exact(refill):
+    """The Tsit5 configuration: rho ~ U[0, 28], 1,024 trajectories, 100 saves on [0, 1]."""
+    y0, p = S.lorenz_rho_ensemble(1024)
+    t = np.linspace(0.0, 1.0, 100)
+    hist, st = tsit5.solve(S.lorenz, y0, t, p, rtol=1e-6, atol=1e-8, return_stats=True,
+                           refill=refill)
+    ref, rst = O.tsit5(O.lorenz(), y0, t, p, rtol=1e-6, atol=1e-8)
+    _same(hist, st, ref, rst)
+
+
+@pytest.mark.parametrize("divergence, golden", [
+    (0.1, (699.81274, 664, 735, 102.78706)),
+    (1.0, (414.56557, 35, 734, 54.40924)),
+])
+def test_lorenz_divergence_is_the_published_distribution_on_gpu(divergence, golden):
+    """benchmarks/nonstiff_lorenz_divergence csv:3, :12 -- 100,000 trajectories."""
+    y0, p = S.lorenz_scenario(100000, 42, divergence=divergence)
+    _, st = tsit5.solve(S.lorenz, y0, np.array(S.LORENZ_TIMES), p, first_step=1e-4, rtol=1e-6,
+                        atol=1e-8, return_stats=True)
+    att = st["accepted_steps"].astype(np.int64) + st["rejected_steps"]
+    assert (att.mean(), att.min(), att.max(), st["rejected_steps"].mean()) == pytest.approx(golden)
+
+
+def test_tsit5_unreached_saves_are_nan_and_counters_match():
+    y0, p = S.lorenz_rho_ensemble(64)
+    t = np.linspace(0.0, 1.0, 50)
+    hist, st = tsit5.solve(S.lorenz, y0, t, p, rtol=1e-6, atol=1e-8, max_steps=5,
+                           return_stats=True)
+    ref, rst = O.tsit5(O.lorenz(), y0, t, p, rtol=1e-6, atol=1e-8, max_steps=5)
+    for k in rst:
+        np.testing.assert_array_equal(st[k], rst[k])
+    np.testing.assert_array_equal(np.isnan(hist), np.isnan(ref))
+    np.testing.assert_array_equal(hist[~np.isnan(hist)], ref[~np.isnan(ref)])
+
+
+def test_tsit5_vdp_ring_twelve_steps():
+    """nonstiff_vdp_divergence csv:2: 64-D ring, mu = 1, 12 steps, none rejected."""
+    fn, _ = S.vdp_lattice(32, mu=1.0)
+    y0, p = S.vdp_scenario(32, 128, 42, divergence=0.0)
+    _, st = tsit5.solve(fn, y0, S.VDP_TIMES, p, first_step=1e-4, rtol=1e-6, atol=1e-8,
+                        return_stats=True)
+    assert (st["loop_steps"] == 12).all() and (st["rejected_steps"] == 0).all()
+
+
+def test_tsit5_backend_names():
+    fn, _ = S.vdp_lattice(16, mu=1.0)
+    y0, p = S.vdp_scenario(16, 8, 42, divergence=1.0)
+    with pytest.raises(ValueError, match="shared backend requires"):
+        tsit5.solve(fn, y0, S.VDP_TIMES, p, backend="shared")
+    fn, _ = S.vdp_lattice(4, mu=1.0)
+    y0, p = S.vdp_scenario(4, 64, 42, divergence=1.0)
+    a = tsit5.solve(fn, y0, S.VDP_TIMES, p, backend="shared", first_step=1e-4, rtol=1e-6, atol=1e-8)
+    b = tsit5.solve(fn, y0, S.VDP_TIMES, p, backend="local", first_step=1e-4, rtol=1e-6, atol=1e-8)
+    np.testing.assert_array_equal(a, b)
