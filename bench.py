@@ -50,6 +50,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "rodas5p_robertson65536_hbm_counters.txt")
+
+
+def profiled_traffic(n):
+    """HBM bytes per launch of the FP32-LU solve kernel, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated); only for the
+    ensemble size that profile was taken at."""
+    if n != N_TRAJ or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
+
+
 def time_solves(run, steps, warmup, torch):
     """Warm up, then time exactly ``steps`` solves bracketed by synchronisation;
     per-launch HIP events on the launch stream give the kernel's own time."""
@@ -95,6 +111,11 @@ def main():
                     help="budget for the CPU oracle baseline sample")
     ap.add_argument("--refill", choices=("auto", "on", "off"), default="auto")
     ap.add_argument("--skip-reference-config", action="store_true")
+    ap.add_argument("--lu", choices=("both", "fp32", "fp64"), default="both",
+                    help="profile runs: time one LU precision only (the JSON line needs both)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="instead of the benchmark, copy MIB MiB device-to-device a few times "
+                         "(a plain copy of known size to scale HBM byte counters by)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(
@@ -110,6 +131,15 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: copied {args.calibrate_copy} MiB three times")
+        return
     arch, n_cu = _runtime.device_info()
     log(f"bench: device {arch}, {n_cu} CUs, N={args.n}")
 
@@ -125,7 +155,7 @@ def main():
     refill = {"auto": None, "on": True, "off": False}[args.refill]
 
     results = {}
-    for lu in ("fp32", "fp64"):
+    for lu in (("fp32", "fp64") if args.lu == "both" else (args.lu,)):
         built = _kernels.rodas5p_kernel(S.robertson, 3, 3, _kernels.KernelOptions(lu_precision=lu))
         info = _runtime.kernel_info(built.plain)
         use_refill = rodas5P._use_refill(built, args.n) if refill is None else refill
@@ -158,6 +188,9 @@ def main():
             f"{args.n / (results[lu]['ms_per_step'] / 1e3):.4g} traj/s, "
             f"mean attempted {attempted / args.n:.2f}")
 
+    if args.lu != "both":
+        log("bench: single-precision profile run, no JSON line")
+        return
     r32, r64 = results["fp32"], results["fp64"]
     traj_s = args.n / (r32["ms_per_step"] / 1e3)
     w = r32["work"]
@@ -206,7 +239,8 @@ def main():
             "peak": roofline.P64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / roofline.P64_TFLOPS,
-            "traffic": None,
+            "traffic": profiled_traffic(args.n),
+            "traffic_source": "profiles/rodas5p_robertson65536_hbm_counters.txt",
             "kernel": "mdx_rodas5p_refill" if r32["refill"] else "mdx_rodas5p",
             "kernel_ms": r32["t_kernel"] * 1e3,
             "flops_fp64": w.fp64,
