@@ -316,63 +316,86 @@ def kaps(n_pairs, epsilon_min):
 # The uneven stiff ensemble: dy/dt = A y + q (y*y), A block-arrow
 # --------------------------------------------------------------------------
 def arrow_structure(n=50, seed=7):
-    """Off-diagonal couplings of the block-arrow matrix and its declared mask.
+    """The block-arrow matrix of the uneven stiff ensemble.
 
-    Variables 0..n-3 pair into 2x2 blocks, variable n-2 stands alone and n-1 is
-    the hub. The coefficients that are actually non-zero keep A lower-arrow
-    triangular -- the upper entry of each 2x2 block and the hub row -- so A's
-    eigenvalues are its diagonal, which the parameters set per trajectory. The
-    declared sparsity mask is the full block arrow (both entries of every block,
-    the hub row and the hub column), a superset of the true pattern as the API
-    allows. Returns ``(rows, cols, vals, mask)``.
+    Variables 0..n-3 pair into 2x2 blocks ``[[a, w], [-w, b]]``, variable n-2
+    stands alone and n-1 is the hub, which every other variable feeds (the hub
+    row is dense; the hub feeds nothing back). The diagonal entries and the
+    block rotation rates ``w`` are per-trajectory parameters: p[0..n-1] the
+    diagonal, p[n + k] the rate of block k. A is block lower triangular, so its
+    eigenvalues are those of the blocks, (a + b)/2 +- sqrt(((a - b)/2)^2 - w^2),
+    and the hub's diagonal. The declared sparsity mask is the full block arrow
+    (hub row and hub column), a superset of the true pattern.
+
+    Returns ``(rows, cols, vals, pidx, mask)`` with one term per off-diagonal
+    entry: ``vals * y[col]``, or ``(vals * p[pidx]) * y[col]`` when pidx >= 0.
     """
     rng = np.random.default_rng(seed)
-    rows, cols, vals = [], [], []
-    for b in range(0, n - 2, 2):
-        rows.append(b)
-        cols.append(b + 1)
-        vals.append(float(rng.uniform(0.5, 1.5)))
+    rows, cols, vals, pidx = [], [], [], []
+    n_blocks = (n - 2) // 2
+    for k in range(n_blocks):
+        b = 2 * k
+        rows += [b, b + 1]
+        cols += [b + 1, b]
+        vals += [1.0, -1.0]
+        pidx += [n + k, n + k]
     for j in range(n - 1):
         rows.append(n - 1)
         cols.append(j)
         vals.append(float(rng.uniform(-0.2, 0.2)))
+        pidx.append(-1)
     mask = np.eye(n, dtype=bool)
-    for b in range(0, n - 2, 2):
+    for k in range(n_blocks):
+        b = 2 * k
         mask[b, b + 1] = mask[b + 1, b] = True
     mask[n - 1, :] = True
     mask[:, n - 1] = True
-    return np.array(rows), np.array(cols), np.array(vals), mask
+    return (np.array(rows), np.array(cols), np.array(vals), np.array(pidx), mask)
 
 
 def arrow_system(n=50, quad=0.01, seed=7):
-    """``ode_fn`` for dy/dt = diag(p) y + offdiag(A) y + quad * y*y."""
-    rows, cols, vals, mask = arrow_structure(n, seed)
+    """``ode_fn`` for dy/dt = A(p) y + quad * y*y over the block arrow; the
+    parameter row has n diagonal entries then (n - 2) // 2 block rates."""
+    rows, cols, vals, pidx, mask = arrow_structure(n, seed)
     terms = [[f"p[{j}] * y[{j}]"] for j in range(n)]
-    for r, c, v in zip(rows, cols, vals):
-        terms[r].append(f"{float(v)!r} * y[{c}]")
+    for r, c, v, k in zip(rows, cols, vals, pidx):
+        if k >= 0:
+            terms[r].append(("" if v > 0 else "-") + f"p[{k}] * y[{c}]")
+        else:
+            terms[r].append(f"{float(v)!r} * y[{c}]")
     components = [
         " + ".join(t) + f" + {float(quad)!r} * (y[{j}] * y[{j}])" for j, t in enumerate(terms)
     ]
     fn = make_tuple_callback("arrow", components)
-    return fn, mask, (rows, cols, vals)
+    return fn, mask, (rows, cols, vals, pidx)
 
 
-def arrow_ensemble(size, n=50, seed=11, lam_range=(1e-1, 1e5), spread=(0.0, 4.0)):
-    """Per-trajectory eigenvalues for the uneven stiff ensemble.
+def arrow_n_params(n=50):
+    return n + (n - 2) // 2
 
-    Each trajectory draws a stiffness ceiling ``10**s`` with ``s`` uniform over
-    ``spread`` (decades above ``lam_range[0]``, capped at ``lam_range[1]``) and
-    then its ``n`` eigenvalues as -LogUniform[lam_range[0], ceiling]; y0 ~ U[-1, 1].
+
+def arrow_ensemble(size, n=50, seed=11, lam_range=(1e-1, 1e5), rate_range=(0.05, 200.0)):
+    """Per-trajectory parameters of the uneven stiff ensemble.
+
+    Real parts: each trajectory draws a stiffness ceiling uniformly between
+    1e1 and 1e5 in log scale and its n diagonal entries as
+    -LogUniform[lam_range[0], ceiling]. Rotation: each trajectory draws one rate
+    scale LogUniform over ``rate_range`` and each block's rate as that scale
+    times U[0.5, 1.5]. The step count of a trajectory grows with the number of
+    oscillations its slowly decaying blocks go through in the window, which is
+    what spreads the ensemble's step counts over two decades. y0 ~ U[-1, 1].
     """
     rng = np.random.default_rng(seed)
+    n_blocks = (n - 2) // 2
     lo = math.log10(lam_range[0])
-    hi_cap = math.log10(lam_range[1])
-    s = rng.uniform(spread[0], spread[1], size=(size, 1))
-    ceiling = np.minimum(lo + 1.0 + s, hi_cap)
+    hi = math.log10(lam_range[1])
+    ceiling = rng.uniform(lo + 2.0, hi, size=(size, 1))
     u = rng.uniform(0.0, 1.0, size=(size, n))
     lam = -(10.0 ** (lo + u * (ceiling - lo)))
+    scale = np.exp(rng.uniform(np.log(rate_range[0]), np.log(rate_range[1]), size=(size, 1)))
+    rates = scale * rng.uniform(0.5, 1.5, size=(size, n_blocks))
     y0 = rng.uniform(-1.0, 1.0, size=(size, n))
-    return y0, lam
+    return y0, np.concatenate([lam, rates], axis=1)
 
 
 def _check_divergence(divergence):

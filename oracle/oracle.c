@@ -129,7 +129,7 @@ enum {
   SYS_CHAIN4 = 4,       /* tests/test_sparsity.py:40-46                       */
   SYS_RING6 = 5,        /* tests/test_sparse_direct.py:333-342                */
   SYS_VDP_LATTICE = 6,  /* reference/systems/python/vdp.py:253-290 (+ open chain) */
-  SYS_LINQUAD = 7,      /* f = diag(p) y + offdiag y + q y*y  (uneven stiff ensemble) */
+  SYS_LINQUAD = 7,      /* f = diag(p) y + off-diagonal terms + q y*y (uneven stiff ensemble) */
   SYS_DAMPED_OSC = 8,   /* tests/test_save_hook.py:23-24                       */
   SYS_KAPS = 9,         /* reference/systems/python/kaps.py:26-48             */
 };
@@ -140,10 +140,12 @@ typedef struct {
   /* lattice */
   int n_osc, wrap, coupling_range;
   double mu, d, omega;
-  /* linquad: off-diagonal triplets and quadratic coefficient */
+  /* linquad: off-diagonal terms and quadratic coefficient; a term is
+   * val * y[col], or (val * p[pidx]) * y[col] when pidx >= 0 */
   int n_off;
   const int* off_row;
   const int* off_col;
+  const int* off_pidx;
   const double* off_val;
   double quad;
   /* kaps: per-pair inverse epsilon */
@@ -222,9 +224,14 @@ static void sys_f(const osys* s, const double* y, double t, const double* p, dou
       break;
     }
     case SYS_LINQUAD: {
+      /* p[j]*y[j] + terms in order + quad*(y[j]*y[j]), then + 0.0 as the
+       * tuple-callback form writes it */
       for (int j = 0; j < s->n; ++j) f[j] = p[j] * y[j];
-      for (int k = 0; k < s->n_off; ++k) f[s->off_row[k]] += s->off_val[k] * y[s->off_col[k]];
-      for (int j = 0; j < s->n; ++j) f[j] += s->quad * (y[j] * y[j]);
+      for (int k = 0; k < s->n_off; ++k) {
+        double c = s->off_pidx[k] >= 0 ? s->off_val[k] * p[s->off_pidx[k]] : s->off_val[k];
+        f[s->off_row[k]] += c * y[s->off_col[k]];
+      }
+      for (int j = 0; j < s->n; ++j) f[j] = (f[j] + s->quad * (y[j] * y[j])) + 0.0;
       break;
     }
     case SYS_DAMPED_OSC:
@@ -300,8 +307,11 @@ static void sys_jac(const osys* s, const double* y, double t, const double* p, d
       break;
     }
     case SYS_LINQUAD:
-      for (int j = 0; j < n; ++j) J[j * n + j] = p[j] + 2.0 * s->quad * y[j];
-      for (int k = 0; k < s->n_off; ++k) J[s->off_row[k] * n + s->off_col[k]] += s->off_val[k];
+      for (int j = 0; j < n; ++j) J[j * n + j] = p[j] + s->quad * (2.0 * y[j]);
+      for (int k = 0; k < s->n_off; ++k) {
+        double c = s->off_pidx[k] >= 0 ? s->off_val[k] * p[s->off_pidx[k]] : s->off_val[k];
+        J[s->off_row[k] * n + s->off_col[k]] += c;
+      }
       break;
     case SYS_DAMPED_OSC:
       J[1] = 1.0;
@@ -784,7 +794,8 @@ static void owork_free(owork* W) {
 
 /* System description passed across the ABI: ints[] and dbls[] as documented in
  * oracle.py (kind, n, n_p, n_osc, wrap, coupling_range, n_off / mu, d, omega,
- * quad), plus the off-diagonal triplets and Kaps inverse epsilons. */
+ * quad), plus the off-diagonal terms (rows, cols, parameter indices) and
+ * the Kaps inverse epsilons. */
 static void unpack_sys(osys* S, const int* si, const double* sd, const int* off_rc,
                        const double* off_val, const double* inv_eps) {
   memset(S, 0, sizeof(*S));
@@ -794,6 +805,7 @@ static void unpack_sys(osys* S, const int* si, const double* sd, const int* off_
   S->mu = sd[0]; S->d = sd[1]; S->omega = sd[2]; S->quad = sd[3];
   S->off_row = off_rc;
   S->off_col = off_rc ? off_rc + S->n_off : NULL;
+  S->off_pidx = off_rc ? off_rc + 2 * S->n_off : NULL;
   S->off_val = off_val;
   S->inv_eps = inv_eps;
 }

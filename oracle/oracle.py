@@ -88,6 +88,7 @@ class System:
     off_rows: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     off_cols: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     off_vals: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float64))
+    off_pidx: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     inv_eps: np.ndarray = field(default_factory=lambda: np.zeros(1, np.float64))
 
     def _pack(self):
@@ -97,8 +98,10 @@ class System:
             dtype=np.int32,
         )
         dbls = np.array([self.mu, self.d, self.omega, self.quad], dtype=np.float64)
+        pidx = self.off_pidx if len(self.off_pidx) == len(self.off_rows) else np.full(
+            len(self.off_rows), -1)
         off_rc = np.ascontiguousarray(
-            np.concatenate([self.off_rows, self.off_cols]).astype(np.int32)
+            np.concatenate([self.off_rows, self.off_cols, pidx]).astype(np.int32)
         )
         if off_rc.size == 0:
             off_rc = np.zeros(1, np.int32)
@@ -145,12 +148,16 @@ def vdp_lattice(n_osc, *, mu=100.0, d=10.0, omega=1.0, coupling_range=1, wrap=Tr
     )
 
 
-def linquad(n, off_rows, off_cols, off_vals, quad):
-    """f = diag(p) y + sum of off-diagonal a_rc y_c + quad * y*y."""
+def linquad(n, off_rows, off_cols, off_vals, quad, off_pidx=None, n_p=None):
+    """f_j = p_j y_j + sum of off-diagonal terms + quad * y_j*y_j, a term being
+    a * y_c, or (a * p_k) * y_c where ``off_pidx`` names a parameter k."""
+    rows = np.asarray(off_rows, np.int32)
     return System(
-        "linquad", n, n, quad=quad,
-        off_rows=np.asarray(off_rows, np.int32), off_cols=np.asarray(off_cols, np.int32),
+        "linquad", n, n if n_p is None else n_p, quad=quad,
+        off_rows=rows, off_cols=np.asarray(off_cols, np.int32),
         off_vals=np.asarray(off_vals, np.float64),
+        off_pidx=np.full(len(rows), -1, np.int32) if off_pidx is None
+        else np.asarray(off_pidx, np.int32),
     )
 
 

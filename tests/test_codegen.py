@@ -48,14 +48,18 @@ def test_rhs_and_jacobian_are_bit_exact_with_the_closed_forms(name, fn, n, n_p, 
         assert np.array_equal(dT, dTo)
 
 
-def test_block_arrow_jacobian_bit_exact():
-    fn, mask, (rows, cols, vals) = S.arrow_system(12)
-    traced = cg.trace(fn, 12, 12)
-    system = O.linquad(12, rows, cols, vals, 0.01)
+def test_block_arrow_rhs_and_jacobian_bit_exact():
+    fn, mask, (rows, cols, vals, pidx) = S.arrow_system(12)
+    n_p = S.arrow_n_params(12)
+    traced = cg.trace(fn, 12, n_p)
+    system = O.linquad(12, rows, cols, vals, 0.01, off_pidx=pidx, n_p=n_p)
     rng = np.random.default_rng(1)
-    y, p = rng.normal(size=12), -rng.uniform(0.1, 100, size=12)
-    J, _ = cg.jacobian_from_graph(traced, y, 0.0, p)
-    assert np.array_equal(J, O.jacobian(system, y, 0.0, p)[0])
+    for _ in range(10):
+        y, p = rng.normal(size=12), rng.normal(size=n_p) * 10
+        assert np.array_equal(cg.rhs_from_graph(traced, y, 0.0, p), O.rhs(system, y, 0.0, p))
+        J, _ = cg.jacobian_from_graph(traced, y, 0.0, p)
+        assert np.array_equal(J, O.jacobian(system, y, 0.0, p)[0])
+    assert (cg.structural_pattern(traced) <= mask).all()
 
 
 def test_coloured_sweeps_recover_the_jacobian():
@@ -168,7 +172,7 @@ def test_rodas5p_sparse_kernel_compiles():
 
     fn, mask, _ = S.arrow_system(50)
     opts = _kernels.KernelOptions(sparsity=normalize_sparsity(mask, 50), unit_weights=False)
-    src, d = _kernels.rodas5p_source(fn, 50, 50, opts)
+    src, d = _kernels.rodas5p_source(fn, 50, S.arrow_n_params(50), opts)
     assert "sparse_factor" in src and d["structure"].size >= int(mask.sum())
     assert _runtime.rtc_compile(src, "arrow.hip")[:4] == b"\x7fELF"
 

@@ -232,15 +232,16 @@ def test_block_arrow_sparse_bit_exact():
     from menodax_amd import _sparse_direct as SD
     from menodax_amd._sparsity import normalize_sparsity
 
-    fn, mask, (rows, cols, vals) = S.arrow_system(50)
+    fn, mask, (rows, cols, vals, pidx) = S.arrow_system(50)
     y0, lam = S.arrow_ensemble(300)
     t = np.linspace(0.0, 10.0, 20)
+    system = O.linquad(50, rows, cols, vals, 0.01, off_pidx=pidx, n_p=S.arrow_n_params(50))
     for lu in ("fp32", "fp64"):
         hist, st = rodas5P.solve(fn, y0, t, lam, sparsity=mask, rtol=1e-6, atol=1e-8,
                                  lu_precision=lu, return_stats=True, refill=True)
         layout = SD.oracle_tables(normalize_sparsity(mask, 50), "amd")
-        ref, rst = O.rodas5p(O.linquad(50, rows, cols, vals, 0.01), y0, t, lam, layout=layout,
-                             rtol=1e-6, atol=1e-8, lu_precision=lu)
+        ref, rst = O.rodas5p(system, y0, t, lam, layout=layout, rtol=1e-6, atol=1e-8,
+                             lu_precision=lu)
         _same(hist, st, ref, rst)
 
 

@@ -234,14 +234,15 @@ def test_dense_lu_against_numpy():
     from (I - h gamma A)^-1; instead check the closed form of the linear system."""
     from scipy.linalg import expm
 
-    rows, cols, vals, _ = S.arrow_structure(10, seed=3)
-    sysd = O.linquad(10, rows, cols, vals, 0.0)
+    rows, cols, vals, pidx, _ = S.arrow_structure(10, seed=3)
+    n_p = S.arrow_n_params(10)
+    sysd = O.linquad(10, rows, cols, vals, 0.0, off_pidx=pidx, n_p=n_p)
     rng = np.random.default_rng(2)
-    lam = -10.0 ** rng.uniform(-1, 2, size=(1, 10))
+    p = np.concatenate([-10.0 ** rng.uniform(-1, 2, size=10), rng.uniform(0.5, 3, size=n_p - 10)])
     y0 = rng.uniform(-1, 1, size=(1, 10))
-    out, _ = O.rodas5p(sysd, y0, np.array([0.0, 0.5]), lam, rtol=1e-11, atol=1e-13,
+    out, _ = O.rodas5p(sysd, y0, np.array([0.0, 0.5]), p[None], rtol=1e-11, atol=1e-13,
                        lu_precision="fp64")
-    A = np.diag(lam[0])
-    for r, c, v in zip(rows, cols, vals):
-        A[r, c] += v
+    A = np.diag(p[:10])
+    for r, c, v, k in zip(rows, cols, vals, pidx):
+        A[r, c] += v * p[k] if k >= 0 else v
     np.testing.assert_allclose(out[0, -1], expm(0.5 * A) @ y0[0], rtol=1e-8, atol=1e-10)

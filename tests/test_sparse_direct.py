@@ -218,7 +218,7 @@ def test_the_compiled_tables_factorise_the_matrix(mask, ordering):
 
 
 @pytest.mark.parametrize("ordering", ["natural", "amd"])
-@pytest.mark.parametrize("mask", MASKS + [S.arrow_structure(50)[3], S.vdp_sparsity(8, wrap=False)],
+@pytest.mark.parametrize("mask", MASKS + [S.arrow_structure(50)[4], S.vdp_sparsity(8, wrap=False)],
                          ids=IDS + ["block-arrow50", "vdp-chain16"])
 def test_the_emitted_device_code_solves_the_matrix(mask, ordering):
     """The unrolled HIP source the kernel compiles, executed statement by statement."""
