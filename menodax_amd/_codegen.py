@@ -758,9 +758,10 @@ class FlopCount:
 class Emitter:
     """Emits straight-line C++ for a set of nodes of one graph.
 
-    Inputs are read once at the top (``y``, ``p`` through the accessor, ``t``),
-    every other node becomes one ``const double`` temporary in graph order, so
-    the source is SSA and the compiler keeps it in registers.
+    Inputs (``y``, ``p`` through the accessor, ``t``) are read once, just
+    before their first use; every other node becomes one ``const double``
+    temporary in graph order, so the source is SSA and the compiler keeps it in
+    registers.
     """
 
     def __init__(self, g: Graph):
@@ -771,18 +772,27 @@ class Emitter:
         lines = []
         flops = 0
         trans = 0
+        # Inputs are read where first used, not all at the top: in a long body
+        # that keeps a state or parameter read from holding a register across
+        # code that does not need it yet.
+        inputs = {"y", "p", "t"}
+        pending = {n.id for n in self.g.nodes if n.id in live and n.op in inputs}
+
+        def read_inputs(ids):
+            for a in ids:
+                if a in pending:
+                    pending.discard(a)
+                    inp = self.g.nodes[a]
+                    expr = {"y": f"y[{inp.value}]", "p": f"p[{inp.value}]", "t": "t"}[inp.op]
+                    lines.append(f"{indent}const double v{a} = {expr};")
+
         for node in self.g.nodes:
-            if node.id not in live:
+            if node.id not in live or node.op in inputs:
                 continue
+            read_inputs([a.id for a in node.args])
             name = f"v{node.id}"
             op = node.op
-            if op == "y":
-                expr = f"y[{node.value}]"
-            elif op == "p":
-                expr = f"p[{node.value}]"
-            elif op == "t":
-                expr = "t"
-            elif op == "c":
+            if op == "c":
                 expr = _lit(node.value)
             elif op in ("add", "sub", "mul", "div"):
                 sym = {"add": "+", "sub": "-", "mul": "*", "div": "/"}[op]
@@ -808,6 +818,7 @@ class Emitter:
             else:  # pragma: no cover
                 raise AssertionError(op)
             lines.append(f"{indent}const double {name} = {expr};")
+        read_inputs(sorted(pending))  # roots that are inputs themselves
         return lines, FlopCount(flops, trans)
 
 
@@ -824,6 +835,7 @@ class GeneratedSystem:
     jac_flops: FlopCount
     n_colours: int
     uses_t: bool
+    dt_zero: bool = False  # df/dt is identically zero (the callback never reads t)
 
 
 def generate_rhs(traced: Traced):
@@ -833,8 +845,8 @@ def generate_rhs(traced: Traced):
     out_lines = [f"  out[{j}] = {value_of(n)};" for j, n in enumerate(traced.outputs)]
     src = "\n".join(
         [
-            "template <class PA>",
-            "__device__ __forceinline__ void rhs(const double* __restrict__ y, const double t,",
+            "template <class PA, class YA>",
+            "__device__ __forceinline__ void rhs(const YA& y, const double t,",
             "                                    const PA& p, double* __restrict__ out) {",
             "  (void)t;",
             "  (void)p;",
@@ -864,8 +876,8 @@ def generate_neg_jac(traced: Traced, *, colour, n_colours, store):
     dT_lines = [f"  dT[{j}] = {value_of(n)};" for j, n in enumerate(dt_tan)]
     src = "\n".join(
         [
-            "template <class PA, class LUT>",
-            "__device__ __forceinline__ void neg_jac(const double* __restrict__ y, const double t,",
+            "template <class PA, class LUT, class YA>",
+            "__device__ __forceinline__ void neg_jac(const YA& y, const double t,",
             "                                        const PA& p, LUT* __restrict__ lu,",
             "                                        double* __restrict__ dT) {",
             "  (void)t;",
@@ -883,12 +895,15 @@ def generate(traced: Traced, *, colour, n_colours, store):
     """``rhs`` and ``neg_jac`` for ``traced`` (see the two generators)."""
     rhs_src, rhs_fc = generate_rhs(traced)
     jac_src, jac_fc = generate_neg_jac(traced, colour=colour, n_colours=n_colours, store=store)
+    g = traced.graph
+    dt_zero = all(n is None for n in tangents(g, traced.outputs, {g.t().id: 1.0}))
     return GeneratedSystem(
         source=rhs_src + "\n\n" + jac_src + "\n",
         rhs_flops=rhs_fc,
         jac_flops=jac_fc,
         n_colours=n_colours,
         uses_t=traced.uses_t,
+        dt_zero=dt_zero,
     )
 
 

@@ -14,6 +14,14 @@
 //   MDX_BLOCK            lanes per workgroup
 //   MDX_PARAMS_IN_REGS   keep the parameter row in registers
 //   MDX_E1/E2/E3         PID exponents
+//   MDX_ROLLED           1: the stage loop is kept as a loop, the stage
+//                        vectors and state live in memory (large n); 0: the
+//                        step is unrolled into registers
+//   MDX_LDS_K, MDX_LDS_Y, MDX_LDS_U   with MDX_ROLLED: the stage vectors, the
+//                        state, the stage argument live in LDS, laid out
+//                        [element][lane]; otherwise k and y are in scratch
+//                        and u in registers
+//   MDX_DT_ZERO          1 when df/dt is identically zero
 //
 // The method, stage by stage, is the reference kernel's (modax/rodas5P.py:
 // tableau 44-105, controller 108-113, 340-342, loop 761-989, tail 991-1005);
@@ -74,10 +82,75 @@ __device__ constexpr double H3[8] = {11.419903575922262, 2.8879645146136994, 72.
                                      80.12511834622643, -52.072871366152654, -59.78993625266729,
                                      -0.15582684282751913, 4.883087185713722};
 
-#if MDX_PARAMS_IN_REGS
+#ifndef MDX_ROLLED
+#define MDX_ROLLED 0
+#endif
+// The rolled kernel copies the parameter row into per-lane scratch: each read
+// is then one load at a constant offset, with no address kept in registers.
+#if MDX_PARAMS_IN_REGS || MDX_ROLLED
 typedef RegParams<MDX_NP> PA;
 #else
 typedef GlobalParams PA;
+#endif
+#ifndef MDX_LDS_K
+#define MDX_LDS_K 0
+#endif
+#ifndef MDX_LDS_Y
+#define MDX_LDS_Y 0
+#endif
+#ifndef MDX_LDS_U
+#define MDX_LDS_U 0
+#endif
+#ifndef MDX_DT_ZERO
+#define MDX_DT_ZERO 0
+#endif
+#if (MDX_LDS_K || MDX_LDS_Y || MDX_LDS_U) && !MDX_ROLLED
+#error "LDS placement needs the rolled stage loop"
+#endif
+
+// Where the stage vectors and the state live. LDS arrays are [element][lane]:
+// a wave's access to one element is 64 consecutive words, conflict-free. The
+// lane index is re-read through an opaque move at the top of each stage, so
+// every LDS address is that one register plus an immediate offset; otherwise
+// the compiler hoists hundreds of per-element addresses out of the step loop
+// and spills them.
+#if MDX_LDS_K || MDX_LDS_Y || MDX_LDS_U
+__device__ __forceinline__ unsigned opaque_lane() {
+  unsigned v = threadIdx.x;
+  asm volatile("; lane" : "+v"(v));
+  return v;
+}
+#define MDX_LANE() const unsigned lane_ = opaque_lane()
+#else
+#define MDX_LANE() (void)0
+#endif
+#if MDX_LDS_K
+__shared__ LUT lds_k[8 * N * MDX_BLOCK];
+#define KQ(q, j) lds_k[((q) * N + (j)) * MDX_BLOCK + lane_]
+#else
+#define KQ(q, j) k[q][j]
+#endif
+#if MDX_LDS_Y
+__shared__ double lds_y[N * MDX_BLOCK];
+#define YJ(st, j) lds_y[(j) * MDX_BLOCK + lane_]
+#else
+#define YJ(st, j) (st).y[j]
+#endif
+#if MDX_LDS_U
+__shared__ double lds_u[N * MDX_BLOCK];
+#define UJ(j) lds_u[(j) * MDX_BLOCK + lane_]
+#else
+#define UJ(j) u[j]
+#endif
+// A vector in LDS as the generated right-hand side reads it.
+struct LdsVec {
+  const double* base;
+  __device__ __forceinline__ double operator[](int j) const { return base[j * MDX_BLOCK]; }
+};
+#if MDX_DT_ZERO
+#define DTJ(j) 0.0
+#else
+#define DTJ(j) dT[j]
 #endif
 
 // ---------------------------------------------------------------------------
@@ -295,18 +368,21 @@ __device__ __forceinline__ void advance(double* __restrict__ u, const double* __
 
 // Everything one trajectory carries from step to step.
 struct State {
+#if !MDX_LDS_Y
   double y[N];
+#endif
   double t, dt, tf, err_prev, err_prev2;
   int save_idx, n_steps, accepted, rejected;
 };
 
 __device__ __forceinline__ void init(State& s, const SolveArgs& a, i64 i, const PA& p) {
+  MDX_LANE();
   const double* __restrict__ times = a.times;
   const double tf = times[a.n_save - 1];
 #pragma unroll
-  for (int j = 0; j < N; ++j) s.y[j] = a.y0[i * a.y0_ts + (i64)j * a.y0_vs];
+  for (int j = 0; j < N; ++j) YJ(s, j) = a.y0[i * a.y0_ts + (i64)j * a.y0_vs];
 #pragma unroll
-  for (int j = 0; j < N; ++j) a.hist[i * a.h_ts + (i64)j * a.h_vs] = s.y[j];
+  for (int j = 0; j < N; ++j) a.hist[i * a.h_ts + (i64)j * a.h_vs] = YJ(s, j);
   s.t = times[0];
   s.dt = a.dt0 > 0.0 ? a.dt0 : (tf - times[0]) * 1e-6;
 #if MDX_TF_INDEX >= 0
@@ -328,10 +404,14 @@ __device__ __forceinline__ bool active(const State& s, const SolveArgs& a) {
 }
 
 // One attempted step, accepted or rejected, with the saves it passes.
-__device__ __forceinline__ void step(State& s, const PA& p, const SolveArgs& a, i64 i) {
+__device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i) {
+#if MDX_ROLLED && !MDX_PARAMS_IN_REGS && MDX_NP > 0
+  if (a.max_steps < 0) p.v[a.n_save % MDX_NP] = 0;  // keeps the row in memory (see below)
+#endif
   double dt_use = s.dt;
   if (dt_use > s.tf - s.t) dt_use = s.tf - s.t;
   if (dt_use < 1e-30) dt_use = 1e-30;
+  MDX_LANE();
   const double inv_dt = 1.0 / dt_use;
   const double t = s.t;
   const double t_end = t + dt_use;
@@ -344,14 +424,114 @@ __device__ __forceinline__ void step(State& s, const PA& p, const SolveArgs& a, 
   for (int q = 0; q < MDX_NLU; ++q) lu[q] = (LUT)0;
 #endif
   double dT[N];
+#if MDX_LDS_Y
+  sys::neg_jac(LdsVec{&lds_y[lane_]}, t, p, lu, dT);
+#else
   sys::neg_jac(s.y, t, p, lu, dT);
+#endif
 #pragma unroll
   for (int d = 0; d < N; ++d) lu[diag_slot(d)] = lu[diag_slot(d)] + (LUT)dtgamma_inv;
   int ipiv[N];
   factor(lu, ipiv);
 
-  double work[N], u[N];
+#if !MDX_LDS_K
   LUT k[8][N];
+#endif
+#if !MDX_LDS_U
+  double u[N];
+#endif
+#if MDX_ROLLED
+  // Large systems: one copy of the stage code, run eight times, so the loop
+  // body fits the instruction cache. The stage vectors and the state live in
+  // LDS or in per-lane scratch (swizzled: one element of a wave is one
+  // contiguous 256-byte access) and the registers hold the matrix and the
+  // stage in flight. For scratch, the stores below are never executed
+  // (max_steps >= 0 is checked before launch); their unresolvable index keeps
+  // each array in memory, and repeating them inside the loop keeps the loads
+  // where they are used rather than hoisted out of the loop.
+#define MDX_PIN(arr) \
+  if (a.max_steps < 0) arr[(a.n_save + S) % N] = 0
+#pragma unroll 1
+  for (int S = 0; S < 8; ++S) {
+    MDX_LANE();
+#if !MDX_LDS_K
+    MDX_PIN(k[S & 7]);
+#endif
+#if !MDX_LDS_Y
+    MDX_PIN(s.y);
+#endif
+#if !MDX_DT_ZERO
+    MDX_PIN(dT);
+#endif
+#if !MDX_PARAMS_IN_REGS && MDX_NP > 0
+    if (a.max_steps < 0) p.v[(a.n_save + S) % MDX_NP] = 0;
+#endif
+    double ts;
+    if (S == 0) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) UJ(j) = YJ(s, j);
+      ts = t;
+    } else if (S < 6) {
+      // u = y + sum_q A_{S,q} k_q (rodas5P.py:649-661)
+      const double* __restrict__ am = AM[S - 1];
+      double acc[N];
+#pragma unroll
+      for (int j = 0; j < N; ++j) acc[j] = am[0] * (double)KQ(0, j);
+#pragma unroll 1
+      for (int q = 1; q < S; ++q) {
+        const double c = am[q];
+#pragma unroll
+        for (int j = 0; j < N; ++j) acc[j] = fmaT(c, (double)KQ(q, j), acc[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < N; ++j) UJ(j) = YJ(s, j) + acc[j];
+      ts = S < 5 ? t + CT[S - 1] * dt_use : t_end;
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) UJ(j) = UJ(j) + (double)KQ(S - 1, j);
+      ts = t_end;
+    }
+    // sum_q C_Sq k_qj, formed before the right-hand side so that the two do
+    // not hold registers at the same time
+    double acc[N];
+    if (S > 0) {
+      const double* __restrict__ cm = CM[S];
+#pragma unroll
+      for (int j = 0; j < N; ++j) acc[j] = cm[0] * (double)KQ(0, j);
+#pragma unroll 1
+      for (int q = 1; q < S; ++q) {
+        const double c = cm[q];
+#pragma unroll
+        for (int j = 0; j < N; ++j) acc[j] = fmaT(c, (double)KQ(q, j), acc[j]);
+      }
+    }
+    double work[N];
+#if MDX_LDS_U
+    sys::rhs(LdsVec{&lds_u[threadIdx.x]}, ts, p, work);
+#else
+    sys::rhs(u, ts, p, work);
+#endif
+    // rhs_j = LU(work_j + dt*d_S*dT_j + (sum_q C_Sq k_qj) / dt) (rodas5P.py:611-647)
+    LUT r[N];
+    if (S == 0) {
+      const double dd = dt_use * DV[0];
+#pragma unroll
+      for (int j = 0; j < N; ++j) r[j] = (LUT)fmaT(dd, DTJ(j), work[j]);
+    } else if (S < 5) {
+      const double dd = dt_use * DV[S];
+#pragma unroll
+      for (int j = 0; j < N; ++j) r[j] = (LUT)fmaT(acc[j], inv_dt, fmaT(dd, DTJ(j), work[j]));
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) r[j] = (LUT)fmaT(acc[j], inv_dt, work[j]);
+    }
+    solve(lu, ipiv, r);
+#pragma unroll
+    for (int j = 0; j < N; ++j) KQ(S, j) = r[j];
+  }
+#undef MDX_PIN
+#else
+  double work[N];
   sys::rhs(s.y, t, p, work);
   stage<0>(work, dT, k, lu, ipiv, dt_use, inv_dt);
   advance<0>(u, s.y, k);
@@ -377,14 +557,15 @@ __device__ __forceinline__ void step(State& s, const PA& p, const SolveArgs& a, 
   for (int j = 0; j < N; ++j) u[j] = u[j] + (double)k[6][j];
   sys::rhs(u, t_end, p, work);
   stage<7>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+#endif
 
   // Weighted RMS error of the embedded estimate k7 (rodas5P.py:903-911).
   double err = 0.0;
 #pragma unroll
   for (int j = 0; j < N; ++j) {
-    const double k7 = (double)k[7][j];
-    const double y_new = u[j] + k7;
-    const double ya = absT(s.y[j]), yb = absT(y_new);
+    const double k7 = (double)KQ(7, j);
+    const double y_new = UJ(j) + k7;
+    const double ya = absT(YJ(s, j)), yb = absT(y_new);
     const double scale = fmaT(a.rtol, ya > yb ? ya : yb, a.atol);
     const double r = weight(a, i, j) * k7 / scale;
     err = fmaT(r, r, err);
@@ -406,27 +587,28 @@ __device__ __forceinline__ void step(State& s, const PA& p, const SolveArgs& a, 
       const double theta1 = 1.0 - theta;
       double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        double h1 = H1[0] * (double)k[0][j];
-        double h2 = H2[0] * (double)k[0][j];
-        double h3 = H3[0] * (double)k[0][j];
+      for (int j = 0; j < N; ++j, row += a.h_vs) {
+        const double k0 = (double)KQ(0, j);
+        double h1 = H1[0] * k0;
+        double h2 = H2[0] * k0;
+        double h3 = H3[0] * k0;
 #pragma unroll
         for (int q = 1; q < 8; ++q) {
-          const double kv = (double)k[q][j];
+          const double kv = (double)KQ(q, j);
           h1 = fmaT(H1[q], kv, h1);
           h2 = fmaT(H2[q], kv, h2);
           h3 = fmaT(H3[q], kv, h3);
         }
-        const double y_new = u[j] + (double)k[7][j];
+        const double y_new = UJ(j) + (double)KQ(7, j);
         double v = fmaT(theta, h3, h2);
         v = fmaT(theta, v, h1);
         v = fmaT(theta1, v, y_new);
-        row[(i64)j * a.h_vs] = fmaT(theta, v, theta1 * s.y[j]);
+        *row = fmaT(theta, v, theta1 * YJ(s, j));
       }
       s.save_idx++;
     }
 #pragma unroll
-    for (int j = 0; j < N; ++j) s.y[j] = u[j] + (double)k[7][j];
+    for (int j = 0; j < N; ++j) YJ(s, j) = UJ(j) + (double)KQ(7, j);
     s.t = t + dt_use;
     s.accepted++;
   } else {
@@ -437,10 +619,11 @@ __device__ __forceinline__ void step(State& s, const PA& p, const SolveArgs& a, 
 
 // Saves past the trajectory's end hold its final state; counters out.
 __device__ __forceinline__ void finish(State& s, const SolveArgs& a, i64 i) {
+  MDX_LANE();
   while (s.save_idx < a.n_save) {
     double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
 #pragma unroll
-    for (int j = 0; j < N; ++j) row[(i64)j * a.h_vs] = s.y[j];
+    for (int j = 0; j < N; ++j) row[(i64)j * a.h_vs] = YJ(s, j);
     s.save_idx++;
   }
   a.accepted[i] = s.accepted;
@@ -452,7 +635,14 @@ __device__ __forceinline__ void finish(State& s, const SolveArgs& a, i64 i) {
 }  // namespace mdx
 
 // One lane per trajectory, every lane stepping until its own trajectory is done.
-extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_rodas5p(mdx::SolveArgs a) {
+// A register cap, when the caller set one (the reference's max_registers).
+#if defined(MDX_MAX_VGPRS) && MDX_MAX_VGPRS > 0
+#define MDX_REG_CAP __attribute__((amdgpu_num_vgpr(MDX_MAX_VGPRS > 256 ? 256 : MDX_MAX_VGPRS)))
+#else
+#define MDX_REG_CAP
+#endif
+
+extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p(mdx::SolveArgs a) {
   using namespace mdx::r5;
   const mdx::i64 i = (mdx::i64)blockIdx.x * MDX_BLOCK + threadIdx.x;
   if (i >= a.n_traj) return;
@@ -470,7 +660,8 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_rodas5p(mdx::SolveAr
 // magnitude. Per trajectory it runs the same init/step/finish as mdx_rodas5p,
 // hence the same bits. Every lane leaves once the counter has passed n_traj
 // and its last trajectory is finished.
-extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_rodas5p_refill(mdx::SolveArgs a) {
+extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p_refill(
+    mdx::SolveArgs a) {
   using namespace mdx::r5;
   PA p;
   State s;

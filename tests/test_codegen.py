@@ -185,3 +185,42 @@ def test_tsit5_kernel_compiles():
 def test_a_compile_error_reports_the_source():
     with pytest.raises(_runtime.NativeError, match="hipRTC compilation"):
         _runtime.rtc_compile("this is not C++", "bad.hip")
+
+
+def test_placement_fills_lds_in_order_of_traffic():
+    P = _kernels.rodas5p_placement
+    assert P(3, "fp32", 64)["placement"] == "registers"
+    assert P(3, "fp32", 64)["MDX_ROLLED"] == 0
+    d = P(50, "fp32", 64)
+    assert (d["MDX_ROLLED"], d["MDX_LDS_K"], d["MDX_LDS_U"], d["MDX_LDS_Y"]) == (1, 1, 1, 1)
+    assert d["lds_bytes"] == (8 * 50 * 4 + 2 * 50 * 8) * 64 <= _kernels.LDS_BYTES_PER_CU
+    # double stage vectors of 50 components do not fit next to anything: state and argument do
+    d = P(50, "fp64", 64)
+    assert (d["MDX_LDS_K"], d["MDX_LDS_U"], d["MDX_LDS_Y"]) == (0, 1, 1)
+    # 64 components: stage vectors and argument fill the 160 KiB exactly, the state stays out
+    d = P(64, "fp32", 64)
+    assert (d["MDX_LDS_K"], d["MDX_LDS_U"], d["MDX_LDS_Y"]) == (1, 1, 0)
+    assert d["lds_bytes"] == _kernels.LDS_BYTES_PER_CU
+    assert P(50, "fp32", 64, "rolled")["lds_bytes"] == 0
+    with pytest.raises(ValueError, match="placement"):
+        P(50, "fp32", 64, "shared")
+
+
+def test_structurally_zero_time_derivative_is_flagged():
+    g = cg.generate(cg.trace(S.robertson, 3, 3), colour=[0, 1, 2], n_colours=3,
+                    store=[(r, c, r * 3 + c) for r in range(3) for c in range(3)])
+    assert g.dt_zero
+    g = cg.generate(cg.trace(S.forced_decay, 1, 2), colour=[0], n_colours=1, store=[(0, 0, 0)])
+    assert not g.dt_zero
+
+
+@pytest.mark.parametrize("placement", ["registers", "rolled", "lds"])
+def test_every_placement_compiles(placement):
+    from menodax_amd._sparsity import normalize_sparsity
+
+    fn, mask, _ = S.arrow_system(12)
+    opts = _kernels.KernelOptions(sparsity=normalize_sparsity(mask, 12), placement=placement,
+                                  max_registers=128)
+    src, d = _kernels.rodas5p_source(fn, 12, S.arrow_n_params(12), opts)
+    assert d["placement"]["placement"] == placement
+    assert _runtime.rtc_compile(src, f"arrow_{placement}.hip")[:4] == b"\x7fELF"

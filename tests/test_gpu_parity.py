@@ -245,6 +245,29 @@ def test_block_arrow_sparse_bit_exact():
         _same(hist, st, ref, rst)
 
 
+@pytest.mark.parametrize("lu", ["fp32", "fp64"])
+@pytest.mark.parametrize("placement", ["registers", "rolled", "lds"])
+def test_every_placement_is_bit_exact(placement, lu):
+    """Unrolled in registers, rolled through scratch, rolled through LDS: the
+    placement moves data, never arithmetic, so each matches the oracle bit for
+    bit in both launch shapes."""
+    from menodax_amd import _sparse_direct as SD
+    from menodax_amd._sparsity import normalize_sparsity
+
+    fn, mask, (rows, cols, vals, pidx) = S.arrow_system(24)
+    n_p = S.arrow_n_params(24)
+    y0, lam = S.arrow_ensemble(200, n=24)
+    t = np.linspace(0.0, 10.0, 7)
+    system = O.linquad(24, rows, cols, vals, 0.01, off_pidx=pidx, n_p=n_p)
+    layout = SD.oracle_tables(normalize_sparsity(mask, 24), "amd")
+    ref, rst = O.rodas5p(system, y0, t, lam, layout=layout, rtol=1e-6, atol=1e-8, lu_precision=lu)
+    for refill in (False, True):
+        hist, st = rodas5P.solve(fn, y0, t, lam, sparsity=mask, rtol=1e-6, atol=1e-8,
+                                 lu_precision=lu, return_stats=True, refill=refill,
+                                 placement=placement)
+        _same(hist, st, ref, rst)
+
+
 # --- Tsit5 --------------------------------------------------------------------
 @pytest.mark.parametrize("refill", [False, True])
 def test_lorenz_rho_ensemble_bit_exact(refill):
