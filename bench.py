@@ -22,7 +22,11 @@ The JSON line also carries
     (rtol 1e-12, FP64) oracle solve on a sample of trajectories;
   - ``reference_config``: the reference's own published benchmark inputs
     (100,000 identical Robertson trajectories, benchmarks/stiff_robertson_scaling)
-    for comparison with its RTX 4070 SUPER numbers.
+    for comparison with its RTX 4070 SUPER numbers;
+  - ``uneven_stiff``: the 100,000-trajectory uneven stiff ensemble (50-variable
+    block-arrow system, step counts spread over two decades): the FP32-LU
+    persistent-refill solve against the plain FP64 one-lane solve, with the
+    endpoint error against FP64 oracle solves.
 
 ``python bench.py --gpus N`` with N > 1 exits: this build runs on one GPU.
 """
@@ -100,6 +104,84 @@ def endpoint_error(hist_gpu, y0, params, times, sample, oracle, O, lu):
             "units": "multiples of atol + rtol*|y_true| at t_end"}
 
 
+def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64):
+    """The uneven stiff ensemble: dy/dt = A y + 0.01 y*y, 50 variables, A block
+    arrow (2x2 rotation blocks plus a dense hub row), eigenvalue real parts
+    -LogUniform down to a per-trajectory stiffness ceiling up to 1e5, t in
+    [0, 10] with 20 saves, rtol 1e-6 atol 1e-8. Times the FP32-LU refill
+    kernel and the plain FP64 kernel (and the other two combinations)."""
+    from menodax_amd import _kernels, _sparse_direct as SD, rodas5P, roofline, systems as S
+    from menodax_amd._sparsity import normalize_sparsity
+
+    n_vars = 50
+    fn, mask, (rows, cols, vals, pidx) = S.arrow_system(n_vars)
+    n_p = S.arrow_n_params(n_vars)
+    sp = normalize_sparsity(mask, n_vars)
+    y0, p = S.arrow_ensemble(n, n=n_vars)
+    times = np.linspace(0.0, 10.0, 20)
+    y0_t = torch.tensor(np.ascontiguousarray(y0.T), device=dev).t()
+    p_t = torch.tensor(np.ascontiguousarray(p.T), device=dev).t()
+    t_t = torch.tensor(times, device=dev)
+    res = {}
+    hists = {}
+    for lu in ("fp32", "fp64"):
+        built = _kernels.rodas5p_kernel(fn, n_vars, n_p, _kernels.KernelOptions(lu_precision=lu,
+                                                                                sparsity=sp))
+        for refill in (True, False):
+            outs = rodas5P.launch(built, y0_t, t_t, p_t, n, dt0=0.0, rtol=RTOL, atol=ATOL,
+                                  max_steps=100000, refill=refill)
+            ws = outs._keep[-1]
+
+            def run(built=built, outs=outs, ws=ws, refill=refill):
+                return rodas5P.launch(built, y0_t, t_t, p_t, n, dt0=0.0, rtol=RTOL, atol=ATOL,
+                                      max_steps=100000, refill=refill, outputs=outs, workspace=ws)
+
+            o, wall, ev_ms = time_solves(run, 2, 0, torch)
+            ms = wall / 2 * 1e3
+            att = (o.accepted + o.rejected).cpu().numpy().astype(np.int64)
+            work = roofline.solve_work(built, int(att.sum()), n * (len(times) - 1))
+            key = f"{lu}_{'refill' if refill else 'plain'}"
+            res[key] = {"ms": ms, "trajectories_per_s": n / (ms / 1e3),
+                        "accepted_steps_per_s": float(o.accepted.sum().item()) / (ms / 1e3),
+                        "roofline_frac": work.roof_seconds() / (ev_ms / 1e3),
+                        "placement": built.extra.get("placement", {}).get("placement")}
+            hists[key] = o.ensemble_view().cpu().numpy()
+            log_fn(f"uneven stiff {key}: {ms:.1f} ms")
+    steps = att
+    edges = np.logspace(np.log10(max(1, steps.min())), np.log10(steps.max() + 1), 9)
+    counts, _ = np.histogram(steps, bins=edges)
+    same = all(np.array_equal(hists[f"{lu}_refill"], hists[f"{lu}_plain"]) for lu in ("fp32", "fp64"))
+    # endpoint error on a sample against the oracle: FP64 LU at the same
+    # tolerance, and a tight FP64 solve
+    idx = np.arange(sample)
+    system = O.linquad(n_vars, rows, cols, vals, 0.01, off_pidx=pidx, n_p=n_p)
+    layout = SD.oracle_tables(sp, "amd")
+    same_tol, _ = O.rodas5p(system, y0[idx], times, p[idx], layout=layout, rtol=RTOL, atol=ATOL,
+                            lu_precision="fp64")
+    tight, _ = O.rodas5p(system, y0[idx], times, p[idx], layout=layout, rtol=1e-10, atol=1e-12,
+                         lu_precision="fp64", max_steps=1000000)
+    err = {}
+    for key in ("fp32_refill", "fp64_plain"):
+        yg = hists[key][idx, -1, :]
+        err[key] = {"max_abs_vs_fp64_oracle_same_tol": float(np.max(np.abs(yg - same_tol[:, -1]))),
+                    "max_abs_vs_tight_oracle": float(np.max(np.abs(yg - tight[:, -1])))}
+    return {
+        "workload": "100,000 trajectories of dy/dt = A y + 0.01 y*y, 50 vars, A block arrow "
+                    "(196 nnz) with per-trajectory stiffness up to 1e5, t in [0,10], 20 saves, "
+                    "rtol 1e-6 atol 1e-8, fp64 state, sparse LU (AMD)",
+        **res,
+        "fp32_refill_over_fp64_plain": res["fp64_plain"]["ms"] / res["fp32_refill"]["ms"],
+        "refill_bit_identical_to_plain": bool(same),
+        "attempted_steps": {"p1": float(np.percentile(steps, 1)),
+                            "median": float(np.median(steps)),
+                            "p99": float(np.percentile(steps, 99)), "max": int(steps.max()),
+                            "p99_over_p1": float(np.percentile(steps, 99) / np.percentile(steps, 1)),
+                            "histogram_edges": [float(e) for e in edges],
+                            "histogram_counts": [int(c) for c in counts]},
+        "endpoint_error": {"sample": sample, **err},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +193,10 @@ def main():
                     help="budget for the CPU oracle baseline sample")
     ap.add_argument("--refill", choices=("auto", "on", "off"), default="auto")
     ap.add_argument("--skip-reference-config", action="store_true")
+    ap.add_argument("--skip-uneven-stiff", action="store_true",
+                    help="leave out the 100k uneven stiff ensemble comparison")
+    ap.add_argument("--only-uneven-stiff", action="store_true",
+                    help="run and print only the uneven stiff ensemble comparison (for profiling)")
     ap.add_argument("--lu", choices=("both", "fp32", "fp64"), default="both",
                     help="profile runs: time one LU precision only (the JSON line needs both)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -128,6 +214,13 @@ def main():
     import torch
 
     from menodax_amd import _kernels, _runtime, roofline, rodas5P, systems as S
+
+    if args.only_uneven_stiff:
+        from oracle import oracle as O
+
+        print(json.dumps({"uneven_stiff": uneven_stiff(torch, torch.device("cuda", 0), O, log)}),
+              flush=True)
+        return
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -317,6 +410,9 @@ def main():
             "published_rtx4070_super_ms": {"fp32": 35.97, "fp64": 45.59},
             **ref,
         }
+
+    if not args.skip_uneven_stiff:
+        line["uneven_stiff"] = uneven_stiff(torch, dev, O, log)
 
     print(json.dumps(line), flush=True)
 

@@ -68,8 +68,10 @@ _CMP = {"lt": "<", "le": "<=", "gt": ">", "ge": ">=", "eq": "==", "ne": "!="}
 
 class Node:
     """One vertex of the expression graph. ``op`` is one of ``y``, ``p``,
-    ``t``, ``c`` (constant), ``add``, ``sub``, ``mul``, ``div``, ``neg``,
-    ``call`` (``fn`` names it), ``cmp`` (``fn`` is the relation) or ``select``."""
+    ``t``, ``in`` (element ``value[1]`` of a named extra input vector
+    ``value[0]``, e.g. a derivative direction), ``c`` (constant), ``add``,
+    ``sub``, ``mul``, ``div``, ``neg``, ``call`` (``fn`` names it), ``cmp``
+    (``fn`` is the relation) or ``select``."""
 
     __slots__ = ("op", "args", "value", "fn", "id")
 
@@ -118,6 +120,9 @@ class Graph:
 
     def t(self):
         return self._make("t")
+
+    def inp(self, name, i):
+        return self._make("in", value=(str(name), int(i)))
 
     # Arithmetic with constant folding. Folding evaluates the same IEEE
     # operation the device would, so it never changes a value.
@@ -501,12 +506,13 @@ def _with_traceable_globals(fn):
 # Forward-mode tangents
 # ---------------------------------------------------------------------------
 def tangents(g: Graph, outputs, seed) -> list:
-    """Directional derivative of ``outputs`` for the seed ``{input node id: 1.0}``.
+    """Directional derivative of ``outputs`` for ``seed = {input node id: d}``,
+    ``d`` either ``1.0`` or a node (a symbolic direction, e.g. an ``in`` input).
 
     Returns one node per output, or ``None`` where the derivative is
-    structurally zero. The rules are the textbook ones; a zero tangent is
-    dropped and a unit one folded, so the emitted code holds only what the
-    colour group actually reaches.
+    structurally zero (``None`` outputs stay ``None``). The rules are the
+    textbook ones; a zero tangent is dropped and a unit one folded, so the
+    emitted code holds only what the seeded inputs actually reach.
     """
     tan: dict[int, Node | None] = {}
     one = g.const(1.0)
@@ -514,13 +520,15 @@ def tangents(g: Graph, outputs, seed) -> list:
     def d(node):
         return tan.get(node.id)
 
-    needed = _ancestors(outputs)
-    for node in g.nodes[: max((n.id for n in outputs), default=-1) + 1]:
+    present = [n for n in outputs if n is not None]
+    needed = _ancestors(present)
+    for node in g.nodes[: max((n.id for n in present), default=-1) + 1]:
         if node.id not in needed:
             continue
         op = node.op
-        if op in ("y", "p", "t"):
-            tan[node.id] = one if seed.get(node.id) else None
+        if op in ("y", "p", "t", "in"):
+            sd = seed.get(node.id)
+            tan[node.id] = sd if isinstance(sd, Node) else (one if sd else None)
             continue
         if op in ("c", "cmp"):
             tan[node.id] = None
@@ -568,7 +576,7 @@ def tangents(g: Graph, outputs, seed) -> list:
             tan[node.id] = _call_tangent(g, node, [d(a) for a in args])
         else:  # pragma: no cover - every op is handled above
             raise AssertionError(op)
-    return [tan.get(n.id) for n in outputs]
+    return [None if n is None else tan.get(n.id) for n in outputs]
 
 
 def _ancestors(outputs):
@@ -679,17 +687,20 @@ def _call_tangent(g, node, dargs):
 # ---------------------------------------------------------------------------
 # Host interpreter (same rounding as the emitted code)
 # ---------------------------------------------------------------------------
-def evaluate(g: Graph, nodes, y, t, p):
+def evaluate(g: Graph, nodes, y, t, p, extra=None):
     """Evaluate ``nodes`` on the host in IEEE double, one rounding per operation
     exactly as the emitted code (compiled with contraction off) performs them.
-    ``None`` entries evaluate to 0.0. Transcendentals use the host libm."""
+    ``None`` entries evaluate to 0.0. Transcendentals use the host libm.
+    ``extra`` maps the names of ``in`` inputs to their vectors."""
     vals: dict[int, float] = {}
     live = _ancestors([n for n in nodes if n is not None])
     for node in g.nodes:
         if node.id not in live:
             continue
         op = node.op
-        if op == "y":
+        if op == "in":
+            v = float(extra[node.value[0]][node.value[1]])
+        elif op == "y":
             v = float(y[node.value])
         elif op == "p":
             v = float(p[node.value])
@@ -775,7 +786,7 @@ class Emitter:
         # Inputs are read where first used, not all at the top: in a long body
         # that keeps a state or parameter read from holding a register across
         # code that does not need it yet.
-        inputs = {"y", "p", "t"}
+        inputs = {"y", "p", "t", "in"}
         pending = {n.id for n in self.g.nodes if n.id in live and n.op in inputs}
 
         def read_inputs(ids):
@@ -783,7 +794,10 @@ class Emitter:
                 if a in pending:
                     pending.discard(a)
                     inp = self.g.nodes[a]
-                    expr = {"y": f"y[{inp.value}]", "p": f"p[{inp.value}]", "t": "t"}[inp.op]
+                    if inp.op == "in":
+                        expr = f"{inp.value[0]}[{inp.value[1]}]"
+                    else:
+                        expr = {"y": f"y[{inp.value}]", "p": f"p[{inp.value}]", "t": "t"}[inp.op]
                     lines.append(f"{indent}const double v{a} = {expr};")
 
         for node in self.g.nodes:
@@ -889,6 +903,89 @@ def generate_neg_jac(traced: Traced, *, colour, n_colours, store):
         + ["}"]
     )
     return src, fc
+
+
+@dataclass
+class SensitivityNodes:
+    """The forward-sensitivity right-hand side as graph nodes.
+
+    With the direction inputs ``s`` (state, n_vars) and ``pd`` (parameters,
+    n_params) and the state increment ``v`` (n_vars):
+
+      rhs[i]      = sum_j df_i/dy_j s_j + sum_m df_i/dp_m pd_m     (J_y s + J_p pd)
+      dt[i]       = d rhs[i] / dt                                   (its df/dt)
+      coupling[i] = sum_j d rhs[i] / dy_j v_j                        (L v)
+
+    ``rhs`` is the sensitivity block of the joint system (modax/_sensitivity.py:
+    198-243), ``dt`` its rows of the joint df/dt (rodas5P.py:400-436) and
+    ``coupling`` the block-triangular stage correction (rodas5P.py:512-561).
+    ``None`` marks a structurally zero entry.
+    """
+
+    rhs: list
+    dt: list
+    coupling: list
+
+
+def sensitivity_nodes(traced: Traced) -> SensitivityNodes:
+    g = traced.graph
+    n, m = traced.n_vars, traced.n_params
+    s = [g.inp("s", j) for j in range(n)]
+    pd = [g.inp("pd", k) for k in range(m)]
+    v = [g.inp("v", j) for j in range(n)]
+    seed = {g.y(j).id: s[j] for j in range(n)}
+    seed.update({g.p(k).id: pd[k] for k in range(m)})
+    rhs = tangents(g, traced.outputs, seed)
+    dt = tangents(g, rhs, {g.t().id: 1.0})
+    coupling = tangents(g, rhs, {g.y(j).id: v[j] for j in range(n)})
+    return SensitivityNodes(rhs, dt, coupling)
+
+
+def augmented_trace(traced: Traced, direction_columns) -> Traced:
+    """The joint ``[y, S]`` system as one traced callback of
+    ``n_vars * (1 + n_dirs)`` components (modax/_sensitivity.py:198-243): for
+    direction ``k`` the block ``z[n + k n : n + (k+1) n]`` is ``S_k`` and its
+    right-hand side ``J_y S_k + J_p e_col`` (no parameter term for ``col < 0``).
+    An explicit method integrates it as it stands."""
+    g = traced.graph
+    n = traced.n_vars
+    outs = list(traced.outputs)
+    zero = g.const(0.0)
+    for k, col in enumerate(direction_columns):
+        seed = {g.y(j).id: g.y(n + k * n + j) for j in range(n)}
+        if col >= 0:
+            seed[g.p(col).id] = 1.0
+        outs += [zero if x is None else x for x in tangents(g, traced.outputs, seed)]
+    return Traced(g, outs, n * (1 + len(direction_columns)), traced.n_params, traced.uses_t)
+
+
+def generate_sensitivity(traced: Traced):
+    """Device source of ``sens_rhs``, ``sens_dt`` and ``sens_coupling`` (see
+    ``SensitivityNodes``); each takes the state through an accessor, ``s``,
+    ``pd`` (and ``v``) through accessors, and writes ``out[0..n_vars)``."""
+    sn = sensitivity_nodes(traced)
+    g = traced.graph
+    parts = []
+    for name, nodes, extra in (("sens_rhs", sn.rhs, ""), ("sens_dt", sn.dt, ""),
+                               ("sens_coupling", sn.coupling, ", const VA& v")):
+        lines, _ = Emitter(g).body(nodes)
+        outs = [f"  out[{j}] = {value_of(x)};" for j, x in enumerate(nodes)]
+        tmpl = "template <class PA, class YA, class SA, class DA" + (", class VA>" if extra else ">")
+        parts.append("\n".join(
+            [tmpl,
+             f"__device__ __forceinline__ void {name}(const YA& y, const double t, const PA& p,",
+             f"    const SA& s, const DA& pd{extra}, double* __restrict__ out) {{",
+             "  (void)t;", "  (void)p;", "  (void)s;", "  (void)pd;"]
+            + lines + outs + ["}"]))
+    return "\n\n".join(parts) + "\n"
+
+
+def sensitivity_from_graph(traced: Traced, y, t, p, s, pd, v=None):
+    """Host evaluation of (rhs, dt, coupling), for tests."""
+    sn = sensitivity_nodes(traced)
+    extra = {"s": s, "pd": pd, "v": np.zeros(traced.n_vars) if v is None else v}
+    ev = lambda nodes: np.array(evaluate(traced.graph, nodes, y, t, p, extra))  # noqa: E731
+    return ev(sn.rhs), ev(sn.dt), ev(sn.coupling)
 
 
 def generate(traced: Traced, *, colour, n_colours, store):

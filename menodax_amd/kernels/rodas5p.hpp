@@ -22,6 +22,11 @@
 //                        [element][lane]; otherwise k and y are in scratch
 //                        and u in registers
 //   MDX_DT_ZERO          1 when df/dt is identically zero
+//   MDX_NSENS            forward-sensitivity directions carried with the state
+//                        (0: a plain solve); the vectors are then
+//                        MDX_N * (1 + MDX_NSENS) long, the matrix stays n x n
+//   MDX_SENS_PCOLS       per direction, the parameter it seeds (-1: a
+//                        direction of the initial state)
 //
 // The method, stage by stage, is the reference kernel's (modax/rodas5P.py:
 // tableau 44-105, controller 108-113, 340-342, loop 761-989, tail 991-1005);
@@ -107,6 +112,33 @@ typedef GlobalParams PA;
 #if (MDX_LDS_K || MDX_LDS_Y || MDX_LDS_U) && !MDX_ROLLED
 #error "LDS placement needs the rolled stage loop"
 #endif
+#ifndef MDX_NSENS
+#define MDX_NSENS 0
+#endif
+#ifndef MDX_SENS_UNROLL
+#define MDX_SENS_UNROLL 1
+#endif
+// Length of every per-trajectory vector: the state, then one block of N per
+// sensitivity direction (modax/_sensitivity.py:24-40).
+constexpr int NS = MDX_NSENS;
+constexpr int NA = N * (1 + NS);
+
+#if MDX_NSENS > 0
+// The parameter each direction seeds, -1 for an initial-state direction.
+__device__ constexpr int PCOL[NS] = {MDX_SENS_PCOLS};
+// A unit parameter direction (or none) as the generated code reads it.
+struct OneHot {
+  int col;
+  __device__ __forceinline__ double operator[](int m) const { return m == col ? 1.0 : 0.0; }
+};
+// Elements off..off+N of a vector, through whatever accessor holds it.
+template <class V>
+struct Shifted {
+  const V& v;
+  int off;
+  __device__ __forceinline__ double operator[](int j) const { return v[off + j]; }
+};
+#endif
 
 // Where the stage vectors and the state live. LDS arrays are [element][lane]:
 // a wave's access to one element is 64 consecutive words, conflict-free. The
@@ -125,19 +157,19 @@ __device__ __forceinline__ unsigned opaque_lane() {
 #define MDX_LANE() (void)0
 #endif
 #if MDX_LDS_K
-__shared__ LUT lds_k[8 * N * MDX_BLOCK];
-#define KQ(q, j) lds_k[((q) * N + (j)) * MDX_BLOCK + lane_]
+__shared__ LUT lds_k[8 * NA * MDX_BLOCK];
+#define KQ(q, j) lds_k[((q) * NA + (j)) * MDX_BLOCK + lane_]
 #else
 #define KQ(q, j) k[q][j]
 #endif
 #if MDX_LDS_Y
-__shared__ double lds_y[N * MDX_BLOCK];
+__shared__ double lds_y[NA * MDX_BLOCK];
 #define YJ(st, j) lds_y[(j) * MDX_BLOCK + lane_]
 #else
 #define YJ(st, j) (st).y[j]
 #endif
 #if MDX_LDS_U
-__shared__ double lds_u[N * MDX_BLOCK];
+__shared__ double lds_u[NA * MDX_BLOCK];
 #define UJ(j) lds_u[(j) * MDX_BLOCK + lane_]
 #else
 #define UJ(j) u[j]
@@ -324,29 +356,90 @@ __device__ __forceinline__ void solve(const LUT* __restrict__ lu, const int* __r
 #endif
 }
 
+// The joint right-hand side at z: the state block, then each direction's
+// J_y s + J_p e (modax/_sensitivity.py:198-243).
+template <class Z>
+__device__ __forceinline__ void rhs_aug(const Z& z, double ts, const PA& p,
+                                        double* __restrict__ work) {
+  sys::rhs(z, ts, p, work);
+#if MDX_NSENS > 0
+#pragma unroll MDX_SENS_UNROLL
+  for (int d = 0; d < NS; ++d)
+    sys::sens_rhs(z, ts, p, Shifted<Z>{z, N + d * N}, OneHot{PCOL[d]}, work + N + d * N);
+#endif
+}
+
+// The sensitivity rows of the joint system's df/dt (rodas5P.py:400-436).
+template <class Z>
+__device__ __forceinline__ void sens_time_derivative(const Z& y, double t, const PA& p,
+                                                     double* __restrict__ dT) {
+#if MDX_NSENS > 0 && !MDX_DT_ZERO
+#pragma unroll MDX_SENS_UNROLL
+  for (int d = 0; d < NS; ++d)
+    sys::sens_dt(y, t, p, Shifted<Z>{y, N + d * N}, OneHot{PCOL[d]}, dT + N + d * N);
+#else
+  (void)y, (void)t, (void)p, (void)dT;
+#endif
+}
+
+// The sensitivity rows of stage S, after its state rows are solved: the joint
+// iteration matrix is block lower triangular with M on every diagonal block, so
+// each direction solves against the same factorisation with its right-hand side
+// corrected by L_d k_y, one second-order sweep at the step's base point
+// (rodas5P.py:512-561). ``sv`` holds the directions' stage values (double),
+// ``ks`` stores into the stage vectors.
+template <class Z, class KS>
+__device__ __forceinline__ void sens_stage(int S, const Z& ybase, double t, const PA& p,
+                                           const double* __restrict__ sv,
+                                           const LUT* __restrict__ lu,
+                                           const int* __restrict__ ipiv, KS ks) {
+#if MDX_NSENS > 0
+  double ky[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) ky[j] = (double)ks(S, j);
+#pragma unroll MDX_SENS_UNROLL
+  for (int d = 0; d < NS; ++d) {
+    double c[N];
+    sys::sens_coupling(ybase, t, p, Shifted<Z>{ybase, N + d * N}, OneHot{PCOL[d]}, ky, c);
+    LUT r[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) r[j] = (LUT)(sv[d * N + j] + c[j]);
+    solve(lu, ipiv, r);
+#pragma unroll
+    for (int j = 0; j < N; ++j) ks(S, N + d * N + j) = r[j];
+  }
+#else
+  (void)S, (void)ybase, (void)t, (void)p, (void)sv, (void)lu, (void)ipiv, (void)ks;
+#endif
+}
+
 // Stage S: rhs_j = LU(work_j + dt*d_S*dT_j + (sum_q C_Sq k_qj) / dt), solved
-// against the step's factorisation into k_S (rodas5P.py:611-647).
+// against the step's factorisation into k_S (rodas5P.py:611-647). Rows past N
+// (sensitivity blocks) keep their value in double for sens_stage.
 template <int S>
 __device__ __forceinline__ void stage(const double* __restrict__ work,
-                                      const double* __restrict__ dT, LUT (*__restrict__ k)[N],
+                                      const double* __restrict__ dT, LUT (*__restrict__ k)[NA],
                                       const LUT* __restrict__ lu, const int* __restrict__ ipiv,
-                                      double dt_use, double inv_dt) {
+                                      double dt_use, double inv_dt, double* __restrict__ sv) {
   LUT r[N];
 #pragma unroll
-  for (int j = 0; j < N; ++j) {
+  for (int j = 0; j < NA; ++j) {
     double v;
     if constexpr (S == 0) {
-      v = fmaT(dt_use * DV[0], dT[j], work[j]);
+      v = fmaT(dt_use * DV[0], DTJ(j), work[j]);
     } else {
       double acc = CM[S][0] * (double)k[0][j];
 #pragma unroll
       for (int q = 1; q < S; ++q) acc = fmaT(CM[S][q], (double)k[q][j], acc);
       if constexpr (S < 5)
-        v = fmaT(acc, inv_dt, fmaT(dt_use * DV[S], dT[j], work[j]));
+        v = fmaT(acc, inv_dt, fmaT(dt_use * DV[S], DTJ(j), work[j]));
       else
         v = fmaT(acc, inv_dt, work[j]);
     }
-    r[j] = (LUT)v;
+    if (j < N)
+      r[j] = (LUT)v;
+    else
+      sv[j - N] = v;
   }
   solve(lu, ipiv, r);
 #pragma unroll
@@ -356,9 +449,9 @@ __device__ __forceinline__ void stage(const double* __restrict__ work,
 // u = y + sum_q A_{S+1,q} k_q (rodas5P.py:649-661).
 template <int S>
 __device__ __forceinline__ void advance(double* __restrict__ u, const double* __restrict__ y,
-                                        LUT (*__restrict__ k)[N]) {
+                                        LUT (*__restrict__ k)[NA]) {
 #pragma unroll
-  for (int j = 0; j < N; ++j) {
+  for (int j = 0; j < NA; ++j) {
     double acc = AM[S][0] * (double)k[0][j];
 #pragma unroll
     for (int q = 1; q <= S; ++q) acc = fmaT(AM[S][q], (double)k[q][j], acc);
@@ -369,7 +462,7 @@ __device__ __forceinline__ void advance(double* __restrict__ u, const double* __
 // Everything one trajectory carries from step to step.
 struct State {
 #if !MDX_LDS_Y
-  double y[N];
+  double y[NA];
 #endif
   double t, dt, tf, err_prev, err_prev2;
   int save_idx, n_steps, accepted, rejected;
@@ -380,9 +473,9 @@ __device__ __forceinline__ void init(State& s, const SolveArgs& a, i64 i, const 
   const double* __restrict__ times = a.times;
   const double tf = times[a.n_save - 1];
 #pragma unroll
-  for (int j = 0; j < N; ++j) YJ(s, j) = a.y0[i * a.y0_ts + (i64)j * a.y0_vs];
+  for (int j = 0; j < NA; ++j) YJ(s, j) = a.y0[i * a.y0_ts + (i64)j * a.y0_vs];
 #pragma unroll
-  for (int j = 0; j < N; ++j) a.hist[i * a.h_ts + (i64)j * a.h_vs] = YJ(s, j);
+  for (int j = 0; j < NA; ++j) a.hist[i * a.h_ts + (i64)j * a.h_vs] = YJ(s, j);
   s.t = times[0];
   s.dt = a.dt0 > 0.0 ? a.dt0 : (tf - times[0]) * 1e-6;
 #if MDX_TF_INDEX >= 0
@@ -416,6 +509,11 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
   const double t = s.t;
   const double t_end = t + dt_use;
   const double dtgamma_inv = 1.0 / (dt_use * GAMMA);
+#if MDX_LDS_Y
+  const LdsVec ybase{&lds_y[lane_]};
+#else
+  const double (&ybase)[NA] = s.y;
+#endif
 
   // M = I/(h gamma) - J straight into the buffer the factorisation works in.
   LUT lu[MDX_NLU];
@@ -423,23 +521,24 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
 #pragma unroll
   for (int q = 0; q < MDX_NLU; ++q) lu[q] = (LUT)0;
 #endif
-  double dT[N];
-#if MDX_LDS_Y
-  sys::neg_jac(LdsVec{&lds_y[lane_]}, t, p, lu, dT);
-#else
-  sys::neg_jac(s.y, t, p, lu, dT);
-#endif
+  double dT[NA];
+  sys::neg_jac(ybase, t, p, lu, dT);
+  sens_time_derivative(ybase, t, p, dT);
 #pragma unroll
   for (int d = 0; d < N; ++d) lu[diag_slot(d)] = lu[diag_slot(d)] + (LUT)dtgamma_inv;
   int ipiv[N];
   factor(lu, ipiv);
 
 #if !MDX_LDS_K
-  LUT k[8][N];
+  LUT k[8][NA];
+  auto ks = [&](int q, int j) -> LUT& { return k[q][j]; };
+#else
+  auto ks = [&](int q, int j) -> LUT& { return KQ(q, j); };
 #endif
 #if !MDX_LDS_U
-  double u[N];
+  double u[NA];
 #endif
+  double sv[NS > 0 ? NA - N : 1];
 #if MDX_ROLLED
   // Large systems: one copy of the stage code, run eight times, so the loop
   // body fits the instruction cache. The stage vectors and the state live in
@@ -449,120 +548,131 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
   // (max_steps >= 0 is checked before launch); their unresolvable index keeps
   // each array in memory, and repeating them inside the loop keeps the loads
   // where they are used rather than hoisted out of the loop.
-#define MDX_PIN(arr) \
-  if (a.max_steps < 0) arr[(a.n_save + S) % N] = 0
+#define MDX_PIN(arr, len) \
+  if (a.max_steps < 0) arr[(a.n_save + S) % (len)] = 0
 #pragma unroll 1
   for (int S = 0; S < 8; ++S) {
     MDX_LANE();
 #if !MDX_LDS_K
-    MDX_PIN(k[S & 7]);
+    MDX_PIN(k[S & 7], NA);
 #endif
 #if !MDX_LDS_Y
-    MDX_PIN(s.y);
+    MDX_PIN(s.y, NA);
 #endif
 #if !MDX_DT_ZERO
-    MDX_PIN(dT);
+    MDX_PIN(dT, NA);
 #endif
 #if !MDX_PARAMS_IN_REGS && MDX_NP > 0
-    if (a.max_steps < 0) p.v[(a.n_save + S) % MDX_NP] = 0;
+    MDX_PIN(p.v, MDX_NP);
 #endif
     double ts;
     if (S == 0) {
 #pragma unroll
-      for (int j = 0; j < N; ++j) UJ(j) = YJ(s, j);
+      for (int j = 0; j < NA; ++j) UJ(j) = YJ(s, j);
       ts = t;
     } else if (S < 6) {
       // u = y + sum_q A_{S,q} k_q (rodas5P.py:649-661)
       const double* __restrict__ am = AM[S - 1];
-      double acc[N];
+      double acc[NA];
 #pragma unroll
-      for (int j = 0; j < N; ++j) acc[j] = am[0] * (double)KQ(0, j);
+      for (int j = 0; j < NA; ++j) acc[j] = am[0] * (double)KQ(0, j);
 #pragma unroll 1
       for (int q = 1; q < S; ++q) {
         const double c = am[q];
 #pragma unroll
-        for (int j = 0; j < N; ++j) acc[j] = fmaT(c, (double)KQ(q, j), acc[j]);
+        for (int j = 0; j < NA; ++j) acc[j] = fmaT(c, (double)KQ(q, j), acc[j]);
       }
 #pragma unroll
-      for (int j = 0; j < N; ++j) UJ(j) = YJ(s, j) + acc[j];
+      for (int j = 0; j < NA; ++j) UJ(j) = YJ(s, j) + acc[j];
       ts = S < 5 ? t + CT[S - 1] * dt_use : t_end;
     } else {
 #pragma unroll
-      for (int j = 0; j < N; ++j) UJ(j) = UJ(j) + (double)KQ(S - 1, j);
+      for (int j = 0; j < NA; ++j) UJ(j) = UJ(j) + (double)KQ(S - 1, j);
       ts = t_end;
     }
     // sum_q C_Sq k_qj, formed before the right-hand side so that the two do
     // not hold registers at the same time
-    double acc[N];
+    double acc[NA];
     if (S > 0) {
       const double* __restrict__ cm = CM[S];
 #pragma unroll
-      for (int j = 0; j < N; ++j) acc[j] = cm[0] * (double)KQ(0, j);
+      for (int j = 0; j < NA; ++j) acc[j] = cm[0] * (double)KQ(0, j);
 #pragma unroll 1
       for (int q = 1; q < S; ++q) {
         const double c = cm[q];
 #pragma unroll
-        for (int j = 0; j < N; ++j) acc[j] = fmaT(c, (double)KQ(q, j), acc[j]);
+        for (int j = 0; j < NA; ++j) acc[j] = fmaT(c, (double)KQ(q, j), acc[j]);
       }
     }
-    double work[N];
+    double work[NA];
 #if MDX_LDS_U
-    sys::rhs(LdsVec{&lds_u[threadIdx.x]}, ts, p, work);
+    rhs_aug(LdsVec{&lds_u[lane_]}, ts, p, work);
 #else
-    sys::rhs(u, ts, p, work);
+    rhs_aug(u, ts, p, work);
 #endif
     // rhs_j = LU(work_j + dt*d_S*dT_j + (sum_q C_Sq k_qj) / dt) (rodas5P.py:611-647)
     LUT r[N];
-    if (S == 0) {
-      const double dd = dt_use * DV[0];
 #pragma unroll
-      for (int j = 0; j < N; ++j) r[j] = (LUT)fmaT(dd, DTJ(j), work[j]);
-    } else if (S < 5) {
-      const double dd = dt_use * DV[S];
-#pragma unroll
-      for (int j = 0; j < N; ++j) r[j] = (LUT)fmaT(acc[j], inv_dt, fmaT(dd, DTJ(j), work[j]));
-    } else {
-#pragma unroll
-      for (int j = 0; j < N; ++j) r[j] = (LUT)fmaT(acc[j], inv_dt, work[j]);
+    for (int j = 0; j < NA; ++j) {
+      double v;
+      if (S == 0)
+        v = fmaT(dt_use * DV[0], DTJ(j), work[j]);
+      else if (S < 5)
+        v = fmaT(acc[j], inv_dt, fmaT(dt_use * DV[S], DTJ(j), work[j]));
+      else
+        v = fmaT(acc[j], inv_dt, work[j]);
+      if (j < N)
+        r[j] = (LUT)v;
+      else
+        sv[j - N] = v;
     }
     solve(lu, ipiv, r);
 #pragma unroll
     for (int j = 0; j < N; ++j) KQ(S, j) = r[j];
+    sens_stage(S, ybase, t, p, sv, lu, ipiv, ks);
   }
 #undef MDX_PIN
 #else
-  double work[N];
-  sys::rhs(s.y, t, p, work);
-  stage<0>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  double work[NA];
+  rhs_aug(s.y, t, p, work);
+  stage<0>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  sens_stage(0, ybase, t, p, sv, lu, ipiv, ks);
   advance<0>(u, s.y, k);
-  sys::rhs(u, t + CT[0] * dt_use, p, work);
-  stage<1>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  rhs_aug(u, t + CT[0] * dt_use, p, work);
+  stage<1>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  sens_stage(1, ybase, t, p, sv, lu, ipiv, ks);
   advance<1>(u, s.y, k);
-  sys::rhs(u, t + CT[1] * dt_use, p, work);
-  stage<2>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  rhs_aug(u, t + CT[1] * dt_use, p, work);
+  stage<2>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  sens_stage(2, ybase, t, p, sv, lu, ipiv, ks);
   advance<2>(u, s.y, k);
-  sys::rhs(u, t + CT[2] * dt_use, p, work);
-  stage<3>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  rhs_aug(u, t + CT[2] * dt_use, p, work);
+  stage<3>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  sens_stage(3, ybase, t, p, sv, lu, ipiv, ks);
   advance<3>(u, s.y, k);
-  sys::rhs(u, t + CT[3] * dt_use, p, work);
-  stage<4>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  rhs_aug(u, t + CT[3] * dt_use, p, work);
+  stage<4>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  sens_stage(4, ybase, t, p, sv, lu, ipiv, ks);
   advance<4>(u, s.y, k);
-  sys::rhs(u, t_end, p, work);
-  stage<5>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  rhs_aug(u, t_end, p, work);
+  stage<5>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  sens_stage(5, ybase, t, p, sv, lu, ipiv, ks);
 #pragma unroll
-  for (int j = 0; j < N; ++j) u[j] = u[j] + (double)k[5][j];
-  sys::rhs(u, t_end, p, work);
-  stage<6>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  for (int j = 0; j < NA; ++j) u[j] = u[j] + (double)k[5][j];
+  rhs_aug(u, t_end, p, work);
+  stage<6>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  sens_stage(6, ybase, t, p, sv, lu, ipiv, ks);
 #pragma unroll
-  for (int j = 0; j < N; ++j) u[j] = u[j] + (double)k[6][j];
-  sys::rhs(u, t_end, p, work);
-  stage<7>(work, dT, k, lu, ipiv, dt_use, inv_dt);
+  for (int j = 0; j < NA; ++j) u[j] = u[j] + (double)k[6][j];
+  rhs_aug(u, t_end, p, work);
+  stage<7>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  sens_stage(7, ybase, t, p, sv, lu, ipiv, ks);
 #endif
 
   // Weighted RMS error of the embedded estimate k7 (rodas5P.py:903-911).
   double err = 0.0;
 #pragma unroll
-  for (int j = 0; j < N; ++j) {
+  for (int j = 0; j < NA; ++j) {
     const double k7 = (double)KQ(7, j);
     const double y_new = UJ(j) + k7;
     const double ya = absT(YJ(s, j)), yb = absT(y_new);
@@ -587,7 +697,7 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
       const double theta1 = 1.0 - theta;
       double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
 #pragma unroll
-      for (int j = 0; j < N; ++j, row += a.h_vs) {
+      for (int j = 0; j < NA; ++j, row += a.h_vs) {
         const double k0 = (double)KQ(0, j);
         double h1 = H1[0] * k0;
         double h2 = H2[0] * k0;
@@ -608,7 +718,7 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
       s.save_idx++;
     }
 #pragma unroll
-    for (int j = 0; j < N; ++j) YJ(s, j) = UJ(j) + (double)KQ(7, j);
+    for (int j = 0; j < NA; ++j) YJ(s, j) = UJ(j) + (double)KQ(7, j);
     s.t = t + dt_use;
     s.accepted++;
   } else {
@@ -623,7 +733,7 @@ __device__ __forceinline__ void finish(State& s, const SolveArgs& a, i64 i) {
   while (s.save_idx < a.n_save) {
     double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
 #pragma unroll
-    for (int j = 0; j < N; ++j) row[(i64)j * a.h_vs] = YJ(s, j);
+    for (int j = 0; j < NA; ++j) row[(i64)j * a.h_vs] = YJ(s, j);
     s.save_idx++;
   }
   a.accepted[i] = s.accepted;
@@ -634,7 +744,6 @@ __device__ __forceinline__ void finish(State& s, const SolveArgs& a, i64 i) {
 }  // namespace r5
 }  // namespace mdx
 
-// One lane per trajectory, every lane stepping until its own trajectory is done.
 // A register cap, when the caller set one (the reference's max_registers).
 #if defined(MDX_MAX_VGPRS) && MDX_MAX_VGPRS > 0
 #define MDX_REG_CAP __attribute__((amdgpu_num_vgpr(MDX_MAX_VGPRS > 256 ? 256 : MDX_MAX_VGPRS)))

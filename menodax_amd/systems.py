@@ -241,6 +241,53 @@ def forced_decay_exact(t, lam, forcing):
     return forcing * (lam * t - 1.0 + np.exp(-lam * t)) / lam**2
 
 
+def cascade(y, t, p):
+    """Linear cascade y0' = -a y0, y1' = a y0 - b y1 (tests/test_sensitivity.py:42-43):
+    closed-form solution and sensitivities in both rates and both initial states."""
+    return (-p[0] * y[0], p[0] * y[0] - p[1] * y[1])
+
+
+def cascade_exact(t, y0, p):
+    """``y(t)`` as ``(n_save, 2)``."""
+    a, b = p
+    c0, c1 = y0
+    k = a / (b - a)
+    return np.stack([c0 * np.exp(-a * t), (c1 - k * c0) * np.exp(-b * t) + k * c0 * np.exp(-a * t)],
+                    axis=-1)
+
+
+def cascade_dparams(t, y0, p):
+    """``dy/d(a, b)`` as ``(n_save, 2, 2)``: differentiate ``cascade_exact``."""
+    a, b = p
+    c0, c1 = y0
+    k = a / (b - a)
+    dk_da, dk_db = b / (b - a) ** 2, -a / (b - a) ** 2
+    ea, eb = np.exp(-a * t), np.exp(-b * t)
+    d_da = np.stack([-t * c0 * ea, c0 * (dk_da * (ea - eb) - k * t * ea)], axis=-1)
+    d_db = np.stack([np.zeros_like(t), c0 * dk_db * (ea - eb) - t * (c1 - k * c0) * eb], axis=-1)
+    return np.stack([d_da, d_db], axis=-1)
+
+
+def cascade_dy0(t, y0, p):
+    """``dy/d(c0, c1)`` as ``(n_save, 2, 2)``."""
+    a, b = p
+    k = a / (b - a)
+    ea, eb = np.exp(-a * t), np.exp(-b * t)
+    return np.stack([np.stack([ea, k * (ea - eb)], axis=-1),
+                     np.stack([np.zeros_like(t), eb], axis=-1)], axis=-1)
+
+
+def forced_decay_dlam(t, lam, forcing):
+    """d/dlam of ``forced_decay_exact``."""
+    return forcing * ((t - t * np.exp(-lam * t)) * lam
+                      - 2.0 * (lam * t - 1.0 + np.exp(-lam * t))) / lam**3
+
+
+def forced_decay_dforcing(t, lam, _forcing):
+    """d/dF of ``forced_decay_exact``."""
+    return (lam * t - 1.0 + np.exp(-lam * t)) / lam**2
+
+
 def decay(y, t, p):
     return (-p[0] * y[0],)
 

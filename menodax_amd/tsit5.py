@@ -11,7 +11,9 @@ check on "shared" is kept so the same calls fail the same way.
 
 from __future__ import annotations
 
-from . import _common, _kernels
+from dataclasses import replace
+
+from . import _autodiff, _common, _kernels
 from ._kernels import Tsit5Options
 from .rodas5P import finish, launch
 
@@ -54,10 +56,9 @@ def solve(
     stream=None,
 ):
     """Integrate an ensemble with Tsit5; see the module docstring."""
-    del sens_error_control, sens_param_columns
     dev = _common.device()
     y0_t, p_t, n, n_vars = _common.normalize_y0_params(y0, params, dev)
-    _check_backend(n_vars, backend)
+    n_params = p_t.shape[1]
     options = Tsit5Options(
         pcoeff=float(pcoeff),
         icoeff=float(icoeff),
@@ -65,14 +66,70 @@ def solve(
         trajectories_per_block=_kernels.trajectories_per_block_or_default(trajectories_per_block),
         unit_weights=error_weights is None,
     )
-    built = _kernels.tsit5_kernel(ode_fn, n_vars, p_t.shape[1], options)
     times = _common.as_f64_device(t_span, dev).contiguous()
     if times.ndim != 1 or times.numel() < 1:
         raise ValueError("t_span must be a non-empty 1-D array of save times")
-    out = launch(
-        built, y0_t, times, p_t, n,
-        dt0=_common.initial_step(first_step), rtol=rtol, atol=atol, max_steps=max_steps,
-        weights=_common.error_weights_soa(error_weights, n, n_vars, dev), refill=refill,
-        stream=stream,
-    )
+    launch_kw = dict(dt0=_common.initial_step(first_step), rtol=rtol, atol=atol,
+                     max_steps=max_steps, refill=refill, stream=stream)
+    wrt_y0, wrt_params = _autodiff.wanted(y0, params)
+    if wrt_y0 or wrt_params:
+        job = _joint_job(ode_fn, options, backend, times, n, n_vars, n_params, error_weights,
+                         dev, launch_kw)
+        state, counters = _autodiff.differentiable_solve(
+            job, y0_t, p_t, wrt_y0, wrt_params, sens_error_control, sens_param_columns)
+        if return_stats:
+            return state, _common.solver_stats(*counters)
+        return state
+    _check_backend(n_vars, backend)
+    built = _kernels.tsit5_kernel(ode_fn, n_vars, n_params, options)
+    out = launch(built, y0_t.detach(), times, p_t.detach(), n,
+                 weights=_common.error_weights_soa(error_weights, n, n_vars, dev), **launch_kw)
     return finish(out, return_stats, _common.wants_numpy(y0, params, t_span))
+
+
+def _joint_job(ode_fn, options, backend, times, n, n_vars, n_params, error_weights, dev,
+               launch_kw):
+    w = None
+    if error_weights is not None:
+        w = _common.as_f64_device(error_weights, dev)
+        w = w.expand(n, n_vars) if w.ndim == 1 else w
+
+    def build(spec, unit_weights):
+        # the joint system is what the backend has to hold (modax/tsit5.py:123-139)
+        _check_backend(spec.n_aug, backend)
+        return _kernels.tsit5_kernel(ode_fn, n_vars, n_params,
+                                     replace(options, sens=spec, unit_weights=unit_weights))
+
+    return _autodiff.JointJob(build=build, times=times, n=n, n_vars=n_vars, n_params=n_params,
+                              weights=w, launch_kw=launch_kw)
+
+
+def solve_sensitivities(ode_fn, y0, t_span, params, *, wrt_y0=True, wrt_params=True,
+                        sens_error_control=True, sens_param_columns=None, return_stats=False,
+                        rtol=1e-8, atol=1e-10, first_step=None, max_steps=100000,
+                        error_weights=None, pcoeff=0.0, icoeff=1.0, dcoeff=0.0, backend="auto",
+                        trajectories_per_block=None, refill=None, stream=None):
+    """One joint ``[y, S]`` Tsit5 solve: the state ``(N, n_save, n_vars)`` and
+    ``S`` ``(N, n_save, n_vars, n_sens)``; see ``rodas5P.solve_sensitivities``."""
+    dev = _common.device()
+    y0_t, p_t, n, n_vars = _common.normalize_y0_params(y0, params, dev)
+    n_params = p_t.shape[1]
+    options = Tsit5Options(
+        pcoeff=float(pcoeff), icoeff=float(icoeff), dcoeff=float(dcoeff),
+        trajectories_per_block=_kernels.trajectories_per_block_or_default(trajectories_per_block),
+        unit_weights=error_weights is None,
+    )
+    times = _common.as_f64_device(t_span, dev).contiguous()
+    launch_kw = dict(dt0=_common.initial_step(first_step), rtol=rtol, atol=atol,
+                     max_steps=max_steps, refill=refill, stream=stream)
+    job = _joint_job(ode_fn, options, backend, times, n, n_vars, n_params, error_weights, dev,
+                     launch_kw)
+    job.spec = _autodiff.SensitivitySpec(
+        n_vars, n_params, bool(wrt_y0), bool(wrt_params), bool(sens_error_control),
+        None if sens_param_columns is None else tuple(int(c) for c in sens_param_columns))
+    out, state, sens = _autodiff.joint_solve(job, y0_t.detach(), p_t.detach())
+    stats = _common.solver_stats(out.accepted, out.rejected, out.loop)
+    if _common.wants_numpy(y0, params, t_span):
+        state, sens = state.cpu().numpy(), sens.cpu().numpy()
+        stats = {k: v.cpu().numpy() for k, v in stats.items()}
+    return (state, sens, stats) if return_stats else (state, sens)

@@ -132,7 +132,22 @@ enum {
   SYS_LINQUAD = 7,      /* f = diag(p) y + off-diagonal terms + q y*y (uneven stiff ensemble) */
   SYS_DAMPED_OSC = 8,   /* tests/test_save_hook.py:23-24                       */
   SYS_KAPS = 9,         /* reference/systems/python/kaps.py:26-48             */
+  SYS_CALLBACK = 10,    /* any system: f, dense J and df/dt from host callbacks */
 };
+
+/* Host callbacks of a SYS_CALLBACK system, and the forward-sensitivity
+ * derivatives of any system a joint solve carries: with a state direction s and
+ * a parameter direction pd,
+ *   sens_f   = J_y s + J_p pd            (the sensitivity rows of the joint rhs)
+ *   sens_dt  = d/dt of sens_f            (their df/dt)
+ *   sens_cpl = D_y(sens_f)[v]            (the block-triangular stage coupling)
+ * modax/_sensitivity.py:148-243, rodas5P.py:400-436, 512-561. */
+typedef void (*cb_f_t)(const double* y, double t, const double* p, double* out);
+typedef void (*cb_jac_t)(const double* y, double t, const double* p, double* J, double* dT);
+typedef void (*cb_sens_t)(const double* y, double t, const double* p, const double* s,
+                          const double* pd, double* out);
+typedef void (*cb_cpl_t)(const double* y, double t, const double* p, const double* s,
+                         const double* pd, const double* v, double* out);
 
 typedef struct {
   int kind;
@@ -150,6 +165,11 @@ typedef struct {
   double quad;
   /* kaps: per-pair inverse epsilon */
   const double* inv_eps;
+  /* callbacks */
+  cb_f_t cb_f;
+  cb_jac_t cb_jac;
+  cb_sens_t cb_sens_f, cb_sens_dt;
+  cb_cpl_t cb_sens_cpl;
 } osys;
 
 /* neighbours along the ring or chain, one entry per place (vdp.py:235-250);
@@ -173,6 +193,9 @@ static int lattice_neighbours(const osys* s, int osc, int* out) {
 
 static void sys_f(const osys* s, const double* y, double t, const double* p, double* f) {
   switch (s->kind) {
+    case SYS_CALLBACK:
+      s->cb_f(y, t, p, f);
+      break;
     case SYS_LORENZ: {
       const double sigma = 10.0, beta = 8.0 / 3.0;
       f[0] = sigma * (y[1] - y[0]);
@@ -259,6 +282,9 @@ static void sys_jac(const osys* s, const double* y, double t, const double* p, d
   memset(J, 0, sizeof(double) * (size_t)n * (size_t)n);
   for (int i = 0; i < n; ++i) dT[i] = 0.0;
   switch (s->kind) {
+    case SYS_CALLBACK:
+      s->cb_jac(y, t, p, J, dT);
+      break;
     case SYS_LORENZ: {
       const double sigma = 10.0, beta = 8.0 / 3.0;
       J[0] = -sigma; J[1] = sigma; J[2] = 0.0;
@@ -542,12 +568,17 @@ typedef struct {
   int lu_fp32;
   double e1, e2, e3;
   int tf_index;
+  /* forward sensitivities: directions, vector width n*(1+n_sens), error divisor,
+   * and per direction the parameter it seeds (-1: an initial-state direction) */
+  int n_sens, na, n_error;
+  const int* pcol;
 } orun;
 
 typedef struct {
   double *y, *u, *work, *dT, *J;
-  double* kd;   /* 8*n, f64 storage */
-  float* ks;    /* 8*n, f32 storage */
+  double *pd, *ky, *cpl; /* sensitivity scratch: n_p, n, n */
+  double* kd;   /* 8*na, f64 storage */
+  float* ks;    /* 8*na, f32 storage */
   double* lud;
   float* lus;
   double* rhsd;
@@ -556,7 +587,24 @@ typedef struct {
 } owork;
 
 static double kget(const orun* R, const owork* W, int s, int j) {
-  return R->lu_fp32 ? (double)W->ks[s * R->n + j] : W->kd[s * R->n + j];
+  return R->lu_fp32 ? (double)W->ks[s * R->na + j] : W->kd[s * R->na + j];
+}
+
+/* unit parameter direction of sensitivity direction d (zeros for -1) */
+static void direction_seed(const orun* R, int d, double* pd) {
+  for (int m = 0; m < R->n_p; ++m) pd[m] = 0.0;
+  if (R->pcol[d] >= 0) pd[R->pcol[d]] = 1.0;
+}
+
+/* the joint right-hand side: state rows, then J_y S_d + J_p e_d per direction */
+static void joint_f(const orun* R, const osys* S, const double* z, double t, const double* p,
+                    double* out, double* pd) {
+  sys_f(S, z, t, p, out);
+  const int n = R->n;
+  for (int d = 0; d < R->n_sens; ++d) {
+    direction_seed(R, d, pd);
+    S->cb_sens_f(z, t, p, z + n + d * n, pd, out + n + d * n);
+  }
 }
 
 static void assemble_and_factor(const orun* R, const olin* L, const osys* S, owork* W,
@@ -564,6 +612,10 @@ static void assemble_and_factor(const orun* R, const olin* L, const osys* S, owo
   const int n = R->n;
   double dtgamma_inv = 1.0 / (dt_use * GAMMA);
   sys_jac(S, W->y, t, p, W->J, W->dT);
+  for (int d = 0; d < R->n_sens; ++d) {
+    direction_seed(R, d, W->pd);
+    S->cb_sens_dt(W->y, t, p, W->y + n + d * n, W->pd, W->dT + n + d * n);
+  }
   if (!L->sparse) {
     if (R->lu_fp32) {
       for (int i = 0; i < n * n; ++i) W->lus[i] = (float)(-W->J[i]);
@@ -595,11 +647,16 @@ static void assemble_and_factor(const orun* R, const olin* L, const osys* S, owo
   }
 }
 
-/* stage s: rhs_j = LU(work_j + dt*d*dT_j + (sum_k C_sk k_kj) * inv_dt), solve, store k_s */
-static void stage(const orun* R, const olin* L, owork* W, int s, const double* crow, double d,
-                  double dt_use, double inv_dt) {
-  const int n = R->n;
-  for (int j = 0; j < n; ++j) {
+/* stage s: rhs_j = LU(work_j + dt*d*dT_j + (sum_k C_sk k_kj) * inv_dt), solve, store k_s.
+ * With sensitivities the rows past n keep their value in work (rodas5P.py:567-583),
+ * and after the state rows are solved each direction solves against the same
+ * factorisation with the coupling L_d k_y added, evaluated at the step's base
+ * point (rodas5P.py:512-561). */
+static void stage(const orun* R, const olin* L, const osys* S, owork* W, int s,
+                  const double* crow, double d, double dt_use, double inv_dt, const double* p,
+                  double t) {
+  const int n = R->n, na = R->na;
+  for (int j = 0; j < na; ++j) {
     double value;
     if (s == 0) {
       value = fma(dt_use * d, W->dT[j], W->work[j]);
@@ -611,19 +668,34 @@ static void stage(const orun* R, const olin* L, owork* W, int s, const double* c
       else
         value = fma(acc, inv_dt, W->work[j]);
     }
-    if (R->lu_fp32) W->rhss[j] = (float)value; else W->rhsd[j] = value;
+    if (j >= n) W->work[j] = value;
+    else if (R->lu_fp32) W->rhss[j] = (float)value;
+    else W->rhsd[j] = value;
   }
-  if (R->lu_fp32) {
-    if (L->sparse) sparse_solve_s(L, W->lus, W->rhss); else dense_solve_s(n, W->lus, W->ipiv, W->rhss);
-    for (int j = 0; j < n; ++j) W->ks[s * n + j] = W->rhss[j];
-  } else {
-    if (L->sparse) sparse_solve_d(L, W->lud, W->rhsd); else dense_solve_d(n, W->lud, W->ipiv, W->rhsd);
-    for (int j = 0; j < n; ++j) W->kd[s * n + j] = W->rhsd[j];
+  for (int blk = 0; blk <= R->n_sens; ++blk) {
+    const int base = blk * n;
+    if (blk > 0) {
+      const int dir = blk - 1;
+      for (int j = 0; j < n; ++j) W->ky[j] = kget(R, W, s, j);
+      direction_seed(R, dir, W->pd);
+      S->cb_sens_cpl(W->y, t, p, W->y + base, W->pd, W->ky, W->cpl);
+      for (int j = 0; j < n; ++j) {
+        double v = W->work[base + j] + W->cpl[j];
+        if (R->lu_fp32) W->rhss[j] = (float)v; else W->rhsd[j] = v;
+      }
+    }
+    if (R->lu_fp32) {
+      if (L->sparse) sparse_solve_s(L, W->lus, W->rhss); else dense_solve_s(n, W->lus, W->ipiv, W->rhss);
+      for (int j = 0; j < n; ++j) W->ks[s * na + base + j] = W->rhss[j];
+    } else {
+      if (L->sparse) sparse_solve_d(L, W->lud, W->rhsd); else dense_solve_d(n, W->lud, W->ipiv, W->rhsd);
+      for (int j = 0; j < n; ++j) W->kd[s * na + base + j] = W->rhsd[j];
+    }
   }
 }
 
 static void advance(const orun* R, owork* W, const double* arow, int n_prev) {
-  for (int j = 0; j < R->n; ++j) {
+  for (int j = 0; j < R->na; ++j) {
     double acc = arow[0] * kget(R, W, 0, j);
     for (int k = 1; k < n_prev; ++k) acc = fma(arow[k], kget(R, W, k, j), acc);
     W->u[j] = W->y[j] + acc;
@@ -633,7 +705,7 @@ static void advance(const orun* R, owork* W, const double* arow, int n_prev) {
 static void rodas5p_one(const orun* R, const olin* L, const osys* S, owork* W, const double* y0,
                         const double* p, const double* w, double* hist, int* acc_out,
                         int* rej_out, int* loop_out) {
-  const int n = R->n, n_save = R->n_save;
+  const int na = R->na, n_save = R->n_save;
   const double* times = R->times;
   const double tf = times[n_save - 1];
   const double dt_init = R->dt0 > 0.0 ? R->dt0 : (tf - times[0]) * 1e-6;
@@ -650,8 +722,8 @@ static void rodas5p_one(const orun* R, const olin* L, const osys* S, owork* W, c
   static const double a3[4] = {A51, A52, A53, A54};
   static const double a4[5] = {A61, A62, A63, A64, A65};
 
-  for (int j = 0; j < n; ++j) W->y[j] = y0[j];
-  for (int j = 0; j < n; ++j) hist[j] = W->y[j];
+  for (int j = 0; j < na; ++j) W->y[j] = y0[j];
+  for (int j = 0; j < na; ++j) hist[j] = W->y[j];
   double t = times[0];
   int save_idx = 1, n_steps = 0, accepted = 0, rejected = 0;
   double dt = dt_init, err_prev = 1.0, err_prev2 = 1.0;
@@ -666,32 +738,32 @@ static void rodas5p_one(const orun* R, const olin* L, const osys* S, owork* W, c
     double t_end = t + dt_use;
     assemble_and_factor(R, L, S, W, p, t, dt_use);
 
-    sys_f(S, W->y, t, p, W->work);
-    stage(R, L, W, 0, NULL, D1, dt_use, inv_dt);
+    joint_f(R, S, W->y, t, p, W->work, W->pd);
+    stage(R, L, S, W, 0, NULL, D1, dt_use, inv_dt, p, t);
     advance(R, W, a0, 1);
-    sys_f(S, W->u, t + RC2 * dt_use, p, W->work);
-    stage(R, L, W, 1, c1, D2, dt_use, inv_dt);
+    joint_f(R, S, W->u, t + RC2 * dt_use, p, W->work, W->pd);
+    stage(R, L, S, W, 1, c1, D2, dt_use, inv_dt, p, t);
     advance(R, W, a1, 2);
-    sys_f(S, W->u, t + RC3 * dt_use, p, W->work);
-    stage(R, L, W, 2, c2, D3, dt_use, inv_dt);
+    joint_f(R, S, W->u, t + RC3 * dt_use, p, W->work, W->pd);
+    stage(R, L, S, W, 2, c2, D3, dt_use, inv_dt, p, t);
     advance(R, W, a2, 3);
-    sys_f(S, W->u, t + RC4 * dt_use, p, W->work);
-    stage(R, L, W, 3, c3, D4, dt_use, inv_dt);
+    joint_f(R, S, W->u, t + RC4 * dt_use, p, W->work, W->pd);
+    stage(R, L, S, W, 3, c3, D4, dt_use, inv_dt, p, t);
     advance(R, W, a3, 4);
-    sys_f(S, W->u, t + RC5 * dt_use, p, W->work);
-    stage(R, L, W, 4, c4, D5, dt_use, inv_dt);
+    joint_f(R, S, W->u, t + RC5 * dt_use, p, W->work, W->pd);
+    stage(R, L, S, W, 4, c4, D5, dt_use, inv_dt, p, t);
     advance(R, W, a4, 5);
-    sys_f(S, W->u, t_end, p, W->work);
-    stage(R, L, W, 5, c5, 0.0, dt_use, inv_dt);
-    for (int j = 0; j < n; ++j) W->u[j] += kget(R, W, 5, j);
-    sys_f(S, W->u, t_end, p, W->work);
-    stage(R, L, W, 6, c6, 0.0, dt_use, inv_dt);
-    for (int j = 0; j < n; ++j) W->u[j] += kget(R, W, 6, j);
-    sys_f(S, W->u, t_end, p, W->work);
-    stage(R, L, W, 7, c7, 0.0, dt_use, inv_dt);
+    joint_f(R, S, W->u, t_end, p, W->work, W->pd);
+    stage(R, L, S, W, 5, c5, 0.0, dt_use, inv_dt, p, t);
+    for (int j = 0; j < na; ++j) W->u[j] += kget(R, W, 5, j);
+    joint_f(R, S, W->u, t_end, p, W->work, W->pd);
+    stage(R, L, S, W, 6, c6, 0.0, dt_use, inv_dt, p, t);
+    for (int j = 0; j < na; ++j) W->u[j] += kget(R, W, 6, j);
+    joint_f(R, S, W->u, t_end, p, W->work, W->pd);
+    stage(R, L, S, W, 7, c7, 0.0, dt_use, inv_dt, p, t);
 
     double err_local = 0.0;
-    for (int j = 0; j < n; ++j) {
+    for (int j = 0; j < na; ++j) {
       double k7 = kget(R, W, 7, j);
       double y_new_j = W->u[j] + k7;
       double a = fabs(W->y[j]), b = fabs(y_new_j);
@@ -699,7 +771,7 @@ static void rodas5p_one(const orun* R, const olin* L, const osys* S, owork* W, c
       double r = w[j] * k7 / scale;
       err_local = fma(r, r, err_local);
     }
-    double err_norm = sqrt(err_local / n);
+    double err_norm = sqrt(err_local / R->n_error);
     int accept = err_norm <= 1.0 && !isnan(err_norm);
     double safe_err;
     if (isnan(err_norm) || err_norm > 1e18) safe_err = 1e18;
@@ -720,7 +792,7 @@ static void rodas5p_one(const orun* R, const olin* L, const osys* S, owork* W, c
         if (!(ts <= t_new + 1e-12 * (1.0 > at ? 1.0 : at))) break;
         double theta = (ts - t_old) / dt_use;
         double theta1 = 1.0 - theta;
-        for (int j = 0; j < n; ++j) {
+        for (int j = 0; j < na; ++j) {
           double h1 = H1[0] * kget(R, W, 0, j), h2 = H2[0] * kget(R, W, 0, j),
                  h3 = H3[0] * kget(R, W, 0, j);
           for (int s = 1; s < 8; ++s) {
@@ -734,11 +806,11 @@ static void rodas5p_one(const orun* R, const olin* L, const osys* S, owork* W, c
           double v = fma(theta, h3, h2);
           v = fma(theta, v, h1);
           v = fma(theta1, v, y_new_j);
-          hist[(size_t)save_idx * n + j] = fma(theta, v, theta1 * W->y[j]);
+          hist[(size_t)save_idx * na + j] = fma(theta, v, theta1 * W->y[j]);
         }
         save_idx++;
       }
-      for (int j = 0; j < n; ++j) W->y[j] = W->u[j] + kget(R, W, 7, j);
+      for (int j = 0; j < na; ++j) W->y[j] = W->u[j] + kget(R, W, 7, j);
       t += dt_use;
       accepted++;
     } else {
@@ -747,7 +819,7 @@ static void rodas5p_one(const orun* R, const olin* L, const osys* S, owork* W, c
     n_steps++;
   }
   while (save_idx < n_save) {
-    for (int j = 0; j < n; ++j) hist[(size_t)save_idx * n + j] = W->y[j];
+    for (int j = 0; j < na; ++j) hist[(size_t)save_idx * na + j] = W->y[j];
     save_idx++;
   }
   *acc_out = accepted;
@@ -772,14 +844,17 @@ static int build_elim_slot(olin* L) {
   return 0;
 }
 
-static void owork_alloc(owork* W, int n, int lu_n) {
-  W->y = (double*)calloc((size_t)n, sizeof(double));
-  W->u = (double*)calloc((size_t)n, sizeof(double));
-  W->work = (double*)calloc((size_t)n, sizeof(double));
-  W->dT = (double*)calloc((size_t)n, sizeof(double));
+static void owork_alloc_aug(owork* W, int n, int na, int n_p, int lu_n) {
+  W->y = (double*)calloc((size_t)na, sizeof(double));
+  W->u = (double*)calloc((size_t)na, sizeof(double));
+  W->work = (double*)calloc((size_t)na, sizeof(double));
+  W->dT = (double*)calloc((size_t)na, sizeof(double));
   W->J = (double*)calloc((size_t)n * (size_t)n, sizeof(double));
-  W->kd = (double*)calloc(8 * (size_t)n, sizeof(double));
-  W->ks = (float*)calloc(8 * (size_t)n, sizeof(float));
+  W->pd = (double*)calloc((size_t)(n_p > 0 ? n_p : 1), sizeof(double));
+  W->ky = (double*)calloc((size_t)n, sizeof(double));
+  W->cpl = (double*)calloc((size_t)n, sizeof(double));
+  W->kd = (double*)calloc(8 * (size_t)na, sizeof(double));
+  W->ks = (float*)calloc(8 * (size_t)na, sizeof(float));
   W->lud = (double*)calloc((size_t)lu_n, sizeof(double));
   W->lus = (float*)calloc((size_t)lu_n, sizeof(float));
   W->rhsd = (double*)calloc((size_t)n, sizeof(double));
@@ -789,6 +864,7 @@ static void owork_alloc(owork* W, int n, int lu_n) {
 
 static void owork_free(owork* W) {
   free(W->y); free(W->u); free(W->work); free(W->dT); free(W->J); free(W->kd); free(W->ks);
+  free(W->pd); free(W->ky); free(W->cpl);
   free(W->lud); free(W->lus); free(W->rhsd); free(W->rhss); free(W->ipiv);
 }
 
@@ -810,24 +886,12 @@ static void unpack_sys(osys* S, const int* si, const double* sd, const int* off_
   S->inv_eps = inv_eps;
 }
 
-/* Ensemble Rodas5P. Arrays are trajectory-major (the reference API's layout):
- * y0 (N,n), params (N,n_p), weights (N,n), hist (N,n_save,n). Returns 0 on
- * success, <0 on a malformed sparse structure. */
-int oracle_rodas5p(const int* sys_i, const double* sys_d, const int* off_rc,
-                   const double* off_val, const double* inv_eps, int N, const double* y0,
-                   const double* times, int n_save, const double* params,
-                   const double* weights, double dt0, double rtol, double atol, int max_steps,
-                   int lu_fp32, double pcoeff, double icoeff, double dcoeff, int tf_index,
-                   /* sparse structure, or nnz <= 0 for dense */
-                   int nnz, const int* row_ptr, const int* col_ind, const int* diag_ptr,
-                   const int* order, const int* slot_of, const int* pattern, double* hist,
-                   int* accepted, int* rejected, int* loop, int n_threads) {
-  osys S;
-  unpack_sys(&S, sys_i, sys_d, off_rc, off_val, inv_eps);
-  const int n = S.n, n_p = S.n_p;
-  orun R = {n, n_p, n_save, times, dt0, rtol, atol, max_steps, lu_fp32,
-            R_EXPONENT * (icoeff + pcoeff + dcoeff), -R_EXPONENT * (pcoeff + 2.0 * dcoeff),
-            R_EXPONENT * dcoeff, tf_index};
+static int run_rodas5p(const osys* S, const orun* R, int N, const double* y0,
+                       const double* params, const double* weights, int nnz,
+                       const int* row_ptr, const int* col_ind, const int* diag_ptr,
+                       const int* order, const int* slot_of, const int* pattern, double* hist,
+                       int* accepted, int* rejected, int* loop, int n_threads) {
+  const int n = R->n, na = R->na, n_p = R->n_p, n_save = R->n_save;
   olin L;
   memset(&L, 0, sizeof(L));
   L.n = n;
@@ -845,19 +909,73 @@ int oracle_rodas5p(const int* sys_i, const double* sys_d, const int* off_rc,
 #endif
   {
     owork W;
-    owork_alloc(&W, n, lu_n);
+    owork_alloc_aug(&W, n, na, n_p, lu_n);
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 8)
 #endif
     for (int i = 0; i < N; ++i) {
-      rodas5p_one(&R, &L, &S, &W, y0 + (size_t)i * n, params + (size_t)i * n_p,
-                  weights + (size_t)i * n, hist + (size_t)i * n_save * n, accepted + i,
+      rodas5p_one(R, &L, S, &W, y0 + (size_t)i * na, params + (size_t)i * n_p,
+                  weights + (size_t)i * na, hist + (size_t)i * n_save * na, accepted + i,
                   rejected + i, loop + i);
     }
     owork_free(&W);
   }
   free(L.elim_slot);
   return 0;
+}
+
+/* Ensemble Rodas5P. Arrays are trajectory-major (the reference API's layout):
+ * y0 (N,n), params (N,n_p), weights (N,n), hist (N,n_save,n). Returns 0 on
+ * success, <0 on a malformed sparse structure. */
+int oracle_rodas5p(const int* sys_i, const double* sys_d, const int* off_rc,
+                   const double* off_val, const double* inv_eps, int N, const double* y0,
+                   const double* times, int n_save, const double* params,
+                   const double* weights, double dt0, double rtol, double atol, int max_steps,
+                   int lu_fp32, double pcoeff, double icoeff, double dcoeff, int tf_index,
+                   /* sparse structure, or nnz <= 0 for dense */
+                   int nnz, const int* row_ptr, const int* col_ind, const int* diag_ptr,
+                   const int* order, const int* slot_of, const int* pattern, double* hist,
+                   int* accepted, int* rejected, int* loop, int n_threads) {
+  osys S;
+  unpack_sys(&S, sys_i, sys_d, off_rc, off_val, inv_eps);
+  const int n = S.n, n_p = S.n_p;
+  orun R = {n, n_p, n_save, times, dt0, rtol, atol, max_steps, lu_fp32,
+            R_EXPONENT * (icoeff + pcoeff + dcoeff), -R_EXPONENT * (pcoeff + 2.0 * dcoeff),
+            R_EXPONENT * dcoeff, tf_index, 0, n, n, NULL};
+  return run_rodas5p(&S, &R, N, y0, params, weights, nnz, row_ptr, col_ind, diag_ptr, order,
+                     slot_of, pattern, hist, accepted, rejected, loop, n_threads);
+}
+
+/* Ensemble Rodas5P of a callback system, optionally carrying n_sens forward
+ * sensitivity directions (the joint [y, S] system of modax/_sensitivity.py):
+ * y0, weights and hist rows are n*(1+n_sens) wide, pcol[d] is the parameter
+ * direction d seeds (-1 for an initial-state direction) and n_error the error
+ * norm's divisor. The callbacks must be reentrant (they run on every thread). */
+int oracle_rodas5p_cb(void* cb_f, void* cb_jac, void* cb_sens_f, void* cb_sens_dt,
+                      void* cb_sens_cpl, int n, int n_p, int n_sens, const int* pcol,
+                      int n_error, int N, const double* y0, const double* times, int n_save,
+                      const double* params, const double* weights, double dt0, double rtol,
+                      double atol, int max_steps, int lu_fp32, double pcoeff, double icoeff,
+                      double dcoeff, int tf_index, int nnz, const int* row_ptr,
+                      const int* col_ind, const int* diag_ptr, const int* order,
+                      const int* slot_of, const int* pattern, double* hist, int* accepted,
+                      int* rejected, int* loop, int n_threads) {
+  osys S;
+  memset(&S, 0, sizeof(S));
+  S.kind = SYS_CALLBACK;
+  S.n = n;
+  S.n_p = n_p;
+  S.cb_f = (cb_f_t)cb_f;
+  S.cb_jac = (cb_jac_t)cb_jac;
+  S.cb_sens_f = (cb_sens_t)cb_sens_f;
+  S.cb_sens_dt = (cb_sens_t)cb_sens_dt;
+  S.cb_sens_cpl = (cb_cpl_t)cb_sens_cpl;
+  if (n_sens > 0 && (!cb_sens_f || !cb_sens_dt || !cb_sens_cpl)) return -10;
+  orun R = {n, n_p, n_save, times, dt0, rtol, atol, max_steps, lu_fp32,
+            R_EXPONENT * (icoeff + pcoeff + dcoeff), -R_EXPONENT * (pcoeff + 2.0 * dcoeff),
+            R_EXPONENT * dcoeff, tf_index, n_sens, n * (1 + n_sens), n_error, pcol};
+  return run_rodas5p(&S, &R, N, y0, params, weights, nnz, row_ptr, col_ind, diag_ptr, order,
+                     slot_of, pattern, hist, accepted, rejected, loop, n_threads);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -23,6 +23,15 @@ Parity status of each oracle function (how it is pinned, see the tests named):
     (benchmarks/nonstiff_lorenz_divergence csv:2) and the seeded divergence
     distributions; VdP 64-D ring 12 steps (nonstiff_vdp_divergence csv:2).
   - ``jacobian``: against central differences of ``rhs``.
+  - ``rodas5p_callbacks`` (a callback system, optionally carrying forward
+    sensitivities): the plain path against ``rodas5p`` on the same system bit
+    for bit; the joint path by closed-form sensitivities (decay, cascade,
+    forced decay: tests/test_sensitivity.py:42-110 of the reference), by
+    central differences of the plain solve, and by the joint solve's state
+    being bit-identical to the plain solve when the sensitivities are kept
+    out of step-size control (modax/_sensitivity.py:116-127, 275-293). The
+    callbacks' derivative code is the code generator's (oracle/callbacks.py),
+    pinned on its own in tests/test_codegen.py.
 """
 
 from __future__ import annotations
@@ -225,6 +234,62 @@ def rodas5p(system, y0, times, params, *, weights=None, first_step=None, rtol=1e
     )
     if rc != 0:
         raise ValueError(f"oracle rejected the sparse structure (code {rc})")
+    return hist, dict(accepted_steps=acc, rejected_steps=rej, loop_steps=loop)
+
+
+def rodas5p_callbacks(cb, y0, times, params, *, direction_columns=(), error_control=True,
+                      weights=None, first_step=None, rtol=1e-8, atol=1e-10, max_steps=100000,
+                      lu_precision="fp32", pcoeff=0.0, icoeff=1.0, dcoeff=0.0, tf_index=None,
+                      layout=None, threads=0):
+    """Ensemble Rodas5P of a ``callbacks.HostCallbacks`` system. With
+    ``direction_columns`` (per direction the parameter it seeds, -1 for an
+    initial-state direction) it integrates the joint ``[y, S]`` system:
+    ``y0`` (and ``weights``) must then be ``n * (1 + n_dirs)`` wide -- the
+    caller builds ``S(t0)`` -- and the history comes back that wide."""
+    lib = _lib()
+    n, n_p = cb.n, cb.n_p
+    cols = np.ascontiguousarray(list(direction_columns) or [0], dtype=np.int32)
+    n_sens = len(direction_columns)
+    na = n * (1 + n_sens)
+    y0 = np.ascontiguousarray(np.atleast_2d(np.asarray(y0, np.float64)))
+    params = np.ascontiguousarray(np.atleast_2d(np.asarray(params, np.float64)))
+    n_traj = max(y0.shape[0], params.shape[0])
+    y0 = np.ascontiguousarray(np.broadcast_to(y0, (n_traj, na)))
+    params = np.ascontiguousarray(np.broadcast_to(params, (n_traj, n_p)))
+    if weights is None:
+        weights = np.ones(na)
+        if not error_control:
+            weights[n:] = 0.0
+    weights = np.ascontiguousarray(np.broadcast_to(np.asarray(weights, np.float64), (n_traj, na)))
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    hist = np.zeros((n_traj, times.size, na))
+    acc = np.zeros(n_traj, np.int32)
+    rej = np.zeros(n_traj, np.int32)
+    loop = np.zeros(n_traj, np.int32)
+    if layout is None:
+        nnz = 0
+        dummy = np.zeros(1, np.int32)
+        row_ptr = col_ind = diag_ptr = order = slot_of = pattern = dummy
+    else:
+        nnz, row_ptr, col_ind, diag_ptr, order, slot_of, pattern = layout
+        row_ptr, col_ind, diag_ptr, order, slot_of, pattern = (
+            np.ascontiguousarray(a, dtype=np.int32)
+            for a in (row_ptr, col_ind, diag_ptr, order, slot_of, pattern)
+        )
+    I, D, V = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    pi = lambda a: _ptr(a, ctypes.c_int)  # noqa: E731
+    pd = lambda a: _ptr(a, ctypes.c_double)  # noqa: E731
+    rc = lib.oracle_rodas5p_cb(
+        *(V(a) for a in cb.addresses), I(n), I(n_p), I(n_sens), pi(cols),
+        I(na if error_control else n), I(n_traj), pd(y0), pd(times), I(times.size), pd(params),
+        pd(weights), D(0.0 if first_step is None else float(first_step)), D(rtol), D(atol),
+        I(max_steps), I(1 if lu_precision == "fp32" else 0), D(pcoeff), D(icoeff), D(dcoeff),
+        I(-1 if tf_index is None else int(tf_index)), I(int(nnz)), pi(row_ptr), pi(col_ind),
+        pi(diag_ptr), pi(order), pi(slot_of), pi(pattern), pd(hist), pi(acc), pi(rej),
+        pi(loop), I(int(threads)),
+    )
+    if rc != 0:
+        raise ValueError(f"oracle rejected the call (code {rc})")
     return hist, dict(accepted_steps=acc, rejected_steps=rej, loop_steps=loop)
 
 
