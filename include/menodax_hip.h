@@ -49,6 +49,7 @@ void mdx_free(void* p);
 /* Kernel flags for mdx_kernel_create. */
 #define MDX_KERNEL_PERSISTENT 1  /* refill kernel; module also holds mdx_reset_queue */
 #define MDX_KERNEL_UNIT_WEIGHTS 2 /* compiled with unit error weights: `weights` unread */
+#define MDX_KERNEL_ADJOINT 4      /* a discrete-adjoint kernel, launched by mdx_adjoint_launch */
 
 /* Register a kernel from a code object: `entry` is its symbol, `block` the
  * workgroup size it was compiled for, `flags` a combination of the
@@ -96,6 +97,32 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
                          double* hist, int32_t* accepted, int32_t* rejected,
                          int32_t* loop_steps, double* hook_out, int32_t hook_size,
                          void* workspace, int64_t workspace_bytes);
+
+/* Device memory mdx_adjoint_launch needs as `workspace`: per trajectory,
+ * max_ckpt checkpoints of n_vars + 8 doubles and ckpt_every step records of
+ * n_vars + 4 doubles. */
+int64_t mdx_adjoint_workspace_bytes(int64_t n_traj, int32_t n_vars, int32_t ckpt_every,
+                                    int32_t max_ckpt);
+
+/* The discrete adjoint of one ensemble solve: given gbar = dL/dhist
+ * (N, n_save, n_vars) for the solve that mdx_solve_launch runs with the same
+ * inputs, write dL/dy0 (N, n_vars) and dL/dparams (N, n_params). The kernel
+ * re-runs the solve storing a checkpoint before every ckpt_every-th accepted
+ * step (at most max_ckpt per trajectory), then goes back through the accepted
+ * steps segment by segment, recomputing each segment from its checkpoint; the
+ * step sizes are held fixed. status[i] is 0, or 1 when trajectory i needed
+ * more than max_ckpt checkpoints (its gradients are then NaN). `layout` as for
+ * mdx_solve_launch (gbar like hist, the gradients like y0 and params).
+ * Replaces the reverse-mode use of the reference's forward-sensitivity JVP rule
+ * (modax/_sensitivity.py:331-413, transposed by JAX): the same gradients for
+ * O(1) memory per step instead of n_vars + n_params sensitivity columns. */
+int32_t mdx_adjoint_launch(int64_t handle, void* stream, int64_t n_traj, int32_t n_vars,
+                           int32_t n_params, int32_t n_save, int32_t layout, const double* y0,
+                           const double* times, const double* params, double dt0, double rtol,
+                           double atol, int32_t max_steps, const double* weights,
+                           const double* gbar, double* grad_y0, double* grad_p, int32_t* status,
+                           int32_t ckpt_every, int32_t max_ckpt, void* workspace,
+                           int64_t workspace_bytes);
 
 /* Device facts used for code generation and the roofline: architecture name
  * (e.g. "gfx950") and compute-unit count of the current device. */

@@ -144,3 +144,102 @@ def differentiable_solve(job: JointJob, y0_t, p_t, wrt_y0, wrt_params, error_con
                                None if param_columns is None else tuple(param_columns))
     state, acc, rej, loop = _FN.apply(y0_t, p_t, job)
     return state, (acc, rej, loop)
+
+
+# --- the discrete adjoint -------------------------------------------------------
+#: accepted steps between checkpoints of the adjoint's forward pass
+DEFAULT_CKPT_EVERY = 64
+
+
+@dataclass
+class AdjointJob:
+    """A plain solve and the adjoint kernel that differentiates it."""
+
+    built: object          # forward kernels (Built)
+    adjoint: object        # adjoint kernel (Built)
+    times: object
+    n: int
+    n_vars: int
+    n_params: int
+    weights: object        # (n_vars, N) error weights or None
+    launch_kw: dict
+    ckpt_every: int = DEFAULT_CKPT_EVERY
+
+
+def adjoint_gradients(job: AdjointJob, y0_t, p_t, gbar, accepted):
+    """dL/dy0 ``(N, n_vars)`` and dL/dparams ``(N, n_params)`` for the cotangent
+    ``gbar`` ``(N, n_save, n_vars)`` of the solve's history; ``accepted`` are
+    that solve's per-trajectory accepted-step counts (they size the
+    checkpoint store)."""
+    import torch
+
+    from . import _runtime
+
+    dev = y0_t.device
+    n, nv, np_ = job.n, job.n_vars, job.n_params
+    c = int(job.ckpt_every)
+    if c < 1:
+        raise ValueError("ckpt_every must be positive")
+    max_ckpt = int(accepted.max().item()) // c + 1 if n else 1
+    ws_bytes = _runtime.adjoint_workspace_bytes(n, nv, c, max_ckpt)
+    ws = torch.empty(max(ws_bytes, 8), dtype=torch.uint8, device=dev)
+    y0_s = _common.soa(y0_t)
+    p_s = _common.soa(p_t) if np_ else torch.zeros((1, max(n, 1)), dtype=torch.float64, device=dev)
+    g_s = gbar.to(device=dev, dtype=torch.float64).permute(1, 2, 0).contiguous()
+    gy = torch.empty((nv, n), dtype=torch.float64, device=dev)
+    gp = torch.empty((max(np_, 1), n), dtype=torch.float64, device=dev)
+    status = torch.empty(n, dtype=torch.int32, device=dev)
+    kw = job.launch_kw
+    stream = kw.get("stream")
+    s = torch.cuda.current_stream(dev) if stream is None else stream
+    _runtime.adjoint_launch(
+        job.adjoint.plain, s.cuda_stream, n, nv, np_, job.times.numel(), _common.LAYOUT_SOA,
+        y0_s.data_ptr(), job.times.data_ptr(), p_s.data_ptr(), kw["dt0"], kw["rtol"], kw["atol"],
+        kw["max_steps"], 0 if job.weights is None else job.weights.data_ptr(), g_s.data_ptr(),
+        gy.data_ptr(), gp.data_ptr(), status.data_ptr(), c, max_ckpt, ws.data_ptr(), ws_bytes)
+    bad = int((status != 0).sum().item())
+    if bad:
+        raise RuntimeError(f"adjoint: {bad} trajectories outran their checkpoint store; "
+                           "their forward pass took more accepted steps than the primal solve")
+    return gy.t(), gp[:np_].t()
+
+
+def _adjoint_function():
+    import torch
+
+    from .rodas5P import launch
+
+    class AdjointSolve(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, y0_t, p_t, job):
+            y0_d, p_d = y0_t.detach(), p_t.detach()
+            out = launch(job.built, y0_d, job.times, p_d, job.n, weights=job.weights,
+                         **job.launch_kw)
+            ctx.job = job
+            ctx.save_for_backward(y0_d, p_d, out.accepted)
+            hist = out.ensemble_view()
+            acc, rej, loop = out.accepted, out.rejected, out.loop
+            ctx.mark_non_differentiable(acc, rej, loop)
+            return hist, acc, rej, loop
+
+        @staticmethod
+        def backward(ctx, g_hist, *_):
+            y0_d, p_d, accepted = ctx.saved_tensors
+            gy, gp = adjoint_gradients(ctx.job, y0_d, p_d, g_hist, accepted)
+            need_y0, need_p = ctx.needs_input_grad[:2]
+            return (gy if need_y0 else None), (gp if need_p else None), None
+
+    return AdjointSolve
+
+
+_ADJ_FN = None
+
+
+def adjoint_solve(job: AdjointJob, y0_t, p_t):
+    """The solve as a differentiable function of ``y0_t``/``p_t`` whose
+    reverse mode runs the discrete-adjoint kernel."""
+    global _ADJ_FN
+    if _ADJ_FN is None:
+        _ADJ_FN = _adjoint_function()
+    hist, acc, rej, loop = _ADJ_FN.apply(y0_t, p_t, job)
+    return hist, (acc, rej, loop)

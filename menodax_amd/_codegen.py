@@ -579,6 +579,60 @@ def tangents(g: Graph, outputs, seed) -> list:
     return [None if n is None else tan.get(n.id) for n in outputs]
 
 
+def adjoints(g: Graph, seeds, wrt) -> list:
+    """Reverse mode: for ``seeds = [(node, weight node), ...]`` the gradient of
+    ``sum weight * node`` with respect to each input node in ``wrt`` (one node
+    per entry, ``None`` where structurally zero). The local partials are the
+    forward rules' (a call's partial is its tangent seeded with 1)."""
+    adj: dict[int, Node | None] = {}
+    one = g.const(1.0)
+    zero = g.const(0.0)
+
+    def acc(node, v):
+        if v is not None:
+            adj[node.id] = _add(g, adj.get(node.id), v)
+
+    present = [(n, w) for n, w in seeds if n is not None and w is not None]
+    for n, w in present:
+        acc(n, w)
+    needed = _ancestors([n for n, _ in present])
+    top = max((n.id for n, _ in present), default=-1)
+    for node in reversed(g.nodes[: top + 1]):
+        if node.id not in needed:
+            continue
+        A = adj.get(node.id)
+        if A is None or node.op in ("y", "p", "t", "in", "c", "cmp"):
+            continue
+        op, args = node.op, node.args
+        if op == "add":
+            acc(args[0], A)
+            acc(args[1], A)
+        elif op == "sub":
+            acc(args[0], A)
+            acc(args[1], g.neg(A))
+        elif op == "mul":
+            acc(args[0], _mul(g, A, args[1]))
+            acc(args[1], _mul(g, A, args[0]))
+        elif op == "div":
+            a, b = args
+            acc(a, g.div(A, b))
+            acc(b, g.neg(g.div(_mul(g, A, node), b)))
+        elif op == "neg":
+            acc(args[0], g.neg(A))
+        elif op == "select":
+            c, x, y = args
+            acc(x, g.select(c, A, zero))
+            acc(y, g.select(c, zero, A))
+        elif op == "call":
+            for k, a in enumerate(args):
+                part = _call_tangent(g, node, [one if j == k else None for j in range(len(args))])
+                if part is not None:
+                    acc(a, _mul(g, part, A))
+        else:  # pragma: no cover
+            raise AssertionError(op)
+    return [adj.get(w.id) for w in wrt]
+
+
 def _ancestors(outputs):
     seen = set()
     stack = list(outputs)
@@ -978,6 +1032,76 @@ def generate_sensitivity(traced: Traced):
              "  (void)t;", "  (void)p;", "  (void)s;", "  (void)pd;"]
             + lines + outs + ["}"]))
     return "\n\n".join(parts) + "\n"
+
+
+@dataclass
+class AdjointNodes:
+    """The derivative contractions a discrete adjoint of Rodas5P needs, as
+    gradients with respect to (y, p) -- ``y_*`` and ``p_*`` lists, ``None``
+    where structurally zero -- of
+
+      f_vjp:   w . f(y, t, p)                (VJP of a stage evaluation)
+      jac_vjp: a . (J_y(y, t, p) b)          (the iteration matrix's dependence on y, p)
+      dt_vjp:  c . df/dt(y, t, p)            (the df/dt term's dependence)
+
+    with ``w``, ``a``, ``b``, ``c`` the named input vectors of n_vars entries."""
+
+    f_y: list
+    f_p: list
+    jac_y: list
+    jac_p: list
+    dt_y: list
+    dt_p: list
+
+
+def adjoint_nodes(traced: Traced) -> AdjointNodes:
+    g = traced.graph
+    n, m = traced.n_vars, traced.n_params
+    ys = [g.y(j) for j in range(n)]
+    ps = [g.p(k) for k in range(m)]
+    w = [g.inp("w", i) for i in range(n)]
+    a = [g.inp("a", i) for i in range(n)]
+    b = [g.inp("b", i) for i in range(n)]
+    c = [g.inp("c", i) for i in range(n)]
+    wrt = ys + ps
+    f = adjoints(g, list(zip(traced.outputs, w)), wrt)
+    jb = tangents(g, traced.outputs, {ys[j].id: b[j] for j in range(n)})
+    jv = adjoints(g, list(zip(jb, a)), wrt)
+    ft = tangents(g, traced.outputs, {g.t().id: 1.0})
+    dv = adjoints(g, list(zip(ft, c)), wrt)
+    return AdjointNodes(f[:n], f[n:], jv[:n], jv[n:], dv[:n], dv[n:])
+
+
+def generate_adjoint(traced: Traced):
+    """Device source of ``f_vjp``, ``jac_vjp`` and ``dt_vjp`` (see
+    ``AdjointNodes``); each adds its gradient into ``ybar`` and ``pbar``."""
+    an = adjoint_nodes(traced)
+    g = traced.graph
+    parts = []
+    for name, extra, ys, ps in (("f_vjp", "const WA& w", an.f_y, an.f_p),
+                                ("jac_vjp", "const WA& a, const WA& b", an.jac_y, an.jac_p),
+                                ("dt_vjp", "const WA& c", an.dt_y, an.dt_p)):
+        lines, _ = Emitter(g).body(ys + ps)
+        outs = [f"  ybar[{j}] += {value_of(x)};" for j, x in enumerate(ys) if x is not None]
+        outs += [f"  pbar[{k}] += {value_of(x)};" for k, x in enumerate(ps) if x is not None]
+        parts.append("\n".join(
+            ["template <class PA, class YA, class WA>",
+             f"__device__ __forceinline__ void {name}(const YA& y, const double t, const PA& p,",
+             f"    {extra}, double* __restrict__ ybar, double* __restrict__ pbar) {{",
+             "  (void)t;", "  (void)p;", "  (void)ybar;", "  (void)pbar;"]
+            + lines + outs + ["}"]))
+    return "\n\n".join(parts) + "\n"
+
+
+def adjoint_from_graph(traced: Traced, y, t, p, *, w=None, a=None, b=None, c=None):
+    """Host evaluation of the three contractions' (ybar, pbar), for tests."""
+    an = adjoint_nodes(traced)
+    z = np.zeros(traced.n_vars)
+    extra = {"w": z if w is None else w, "a": z if a is None else a,
+             "b": z if b is None else b, "c": z if c is None else c}
+    ev = lambda nodes: np.array(evaluate(traced.graph, nodes, y, t, p, extra))  # noqa: E731
+    return {"f": (ev(an.f_y), ev(an.f_p)), "jac": (ev(an.jac_y), ev(an.jac_p)),
+            "dt": (ev(an.dt_y), ev(an.dt_p))}
 
 
 def sensitivity_from_graph(traced: Traced, y, t, p, s, pd, v=None):

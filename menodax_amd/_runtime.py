@@ -23,6 +23,7 @@ ARCH = "gfx950"
 
 KERNEL_PERSISTENT = 1
 KERNEL_UNIT_WEIGHTS = 2
+KERNEL_ADJOINT = 4
 
 # Every kernel is compiled with contraction off (the fused multiply-adds are
 # explicit in the source) and without packed FP32 instructions.
@@ -89,6 +90,13 @@ def lib():
             vp, vp, vp, vp, vp, i32, vp, i64,
         ]
         L.mdx_solve_launch.restype = i32
+        L.mdx_adjoint_workspace_bytes.argtypes = [i64, i32, i32, i32]
+        L.mdx_adjoint_workspace_bytes.restype = i64
+        L.mdx_adjoint_launch.argtypes = [
+            i64, vp, i64, i32, i32, i32, i32, vp, vp, vp, dbl, dbl, dbl, i32, vp,
+            vp, vp, vp, vp, i32, i32, vp, i64,
+        ]
+        L.mdx_adjoint_launch.restype = i32
         L.mdx_device_info.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(i32)]
         L.mdx_device_info.restype = i32
         if L.mdx_abi_version() != 1:
@@ -247,6 +255,28 @@ def device_info():
     n_cu = ctypes.c_int32()
     _check(lib().mdx_device_info(buf, 128, ctypes.byref(n_cu)), "mdx_device_info")
     return buf.value.decode(), int(n_cu.value)
+
+
+def adjoint_workspace_bytes(n_traj, n_vars, ckpt_every, max_ckpt) -> int:
+    v = lib().mdx_adjoint_workspace_bytes(int(n_traj), int(n_vars), int(ckpt_every), int(max_ckpt))
+    if v < 0:
+        raise ValueError("mdx_adjoint_workspace_bytes: bad extents")
+    return int(v)
+
+
+def adjoint_launch(handle, stream, n_traj, n_vars, n_params, n_save, layout, y0, times, params,
+                   dt0, rtol, atol, max_steps, weights, gbar, grad_y0, grad_p, status,
+                   ckpt_every, max_ckpt, workspace, workspace_bytes):
+    """Launch through ``mdx_adjoint_launch``; pointers are plain integers."""
+    _check(
+        lib().mdx_adjoint_launch(
+            handle, stream, int(n_traj), int(n_vars), int(n_params), int(n_save), int(layout),
+            y0, times, params or None, float(dt0), float(rtol), float(atol), int(max_steps),
+            weights or None, gbar, grad_y0, grad_p or None, status, int(ckpt_every),
+            int(max_ckpt), workspace, int(workspace_bytes),
+        ),
+        "mdx_adjoint_launch",
+    )
 
 
 def solve_launch(handle, stream, grid, n_traj, n_vars, n_params, n_save, layout, y0, times, params,

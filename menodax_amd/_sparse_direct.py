@@ -197,6 +197,7 @@ class SparseDirectSolver:
         parts = [f"__device__ constexpr int DIAG[{self.layout.n_vars}] = {{{diag}}};"]
         parts.append(_factor_source(self.layout, self.ops, unroll_factor))
         parts.append(_solve_source(self.layout, unroll_solve))
+        parts.append(_solve_t_source(self.layout, unroll_solve))
         return "\n".join(parts)
 
     def __repr__(self):
@@ -321,6 +322,56 @@ def _solve_source(layout, unrolled: bool) -> str:
         f"    for (int p = pivot + 1; p < SOLVE_TAB[{ROW} + i + 1]; ++p)",
         f"      acc = mdx::fmaT(-lu[p], r[SOLVE_TAB[{CORIG} + p]], acc);",
         "    r[row] = acc * lu[pivot];",
+        "  }",
+        "}",
+    ])
+
+
+def _solve_t_source(layout, unrolled: bool) -> str:
+    """The transposed solve ``M^T x = b`` with the same factors (M = L U in
+    elimination order): ``U^T z = b`` forward, each row's pivot scaled by the
+    stored inverse diagonal and its upper entries scattered onward, then
+    ``L^T x = z`` backward, scattering the lower entries. For the adjoint."""
+    n = layout.n_vars
+    row_ptr, diag_ptr = layout.row_ptr, layout.diag_ptr
+    origin, col_origin = layout.row_origin, layout.col_origin
+    if unrolled:
+        lines = ["template <class T>",
+                 "__device__ __forceinline__ void sparse_solve_t(const T* __restrict__ lu, "
+                 "T* __restrict__ r) {"]
+        for i in range(n):
+            row, piv = int(origin[i]), int(diag_ptr[i])
+            body = "".join(f" r[{int(col_origin[p])}] = mdx::fmaT(-lu[{p}], z, r[{int(col_origin[p])}]);"
+                           for p in range(piv + 1, int(row_ptr[i + 1])))
+            lines.append(f"  {{ const T z = r[{row}] * lu[{piv}]; r[{row}] = z;{body} }}")
+        for i in range(n - 1, -1, -1):
+            row = int(origin[i])
+            a, b = int(row_ptr[i]), int(diag_ptr[i])
+            if a == b:
+                continue
+            body = "".join(f" r[{int(col_origin[p])}] = mdx::fmaT(-lu[{p}], x, r[{int(col_origin[p])}]);"
+                           for p in range(a, b))
+            lines.append(f"  {{ const T x = r[{row}];{body} }}")
+        lines.append("}")
+        return "\n".join(lines)
+    tabs, (ROW, DIAG_, ORIG, CORIG) = _pack(row_ptr, diag_ptr, origin, col_origin)
+    return "\n".join([
+        _const_table("SOLVE_T_TAB", tabs),
+        "template <class T>",
+        "__device__ __forceinline__ void sparse_solve_t(const T* __restrict__ lu, T* __restrict__ r) {",
+        f"  for (int i = 0; i < {n}; ++i) {{",
+        f"    const int row = SOLVE_T_TAB[{ORIG} + i];",
+        f"    const int pivot = SOLVE_T_TAB[{DIAG_} + i];",
+        "    const T z = r[row] * lu[pivot];",
+        "    r[row] = z;",
+        f"    for (int p = pivot + 1; p < SOLVE_T_TAB[{ROW} + i + 1]; ++p)",
+        f"      r[SOLVE_T_TAB[{CORIG} + p]] = mdx::fmaT(-lu[p], z, r[SOLVE_T_TAB[{CORIG} + p]]);",
+        "  }",
+        f"  for (int i = {n - 1}; i >= 0; --i) {{",
+        f"    const int row = SOLVE_T_TAB[{ORIG} + i];",
+        "    const T x = r[row];",
+        f"    for (int p = SOLVE_T_TAB[{ROW} + i]; p < SOLVE_T_TAB[{DIAG_} + i]; ++p)",
+        f"      r[SOLVE_T_TAB[{CORIG} + p]] = mdx::fmaT(-lu[p], x, r[SOLVE_T_TAB[{CORIG} + p]]);",
         "  }",
         "}",
     ])

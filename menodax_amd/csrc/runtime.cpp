@@ -52,6 +52,20 @@ struct SolveArgs {
 };
 static_assert(sizeof(SolveArgs) == 10 * 8 + 3 * 8 + 8 + 9 * 8 + 8, "SolveArgs layout");
 
+// Mirrors mdx::r5::AdjointArgs in kernels/rodas5p_adjoint.hpp.
+struct AdjointArgs {
+  SolveArgs s;
+  const double* gbar;
+  double* grad_y0;
+  double* grad_p;
+  int32_t* status;
+  double* ws;
+  int64_t g_ts, g_ss, g_vs;
+  int64_t gy_ts, gy_vs, gp_ts, gp_vs;
+  int32_t ckpt_every, max_ckpt;
+};
+static_assert(sizeof(AdjointArgs) == sizeof(SolveArgs) + 5 * 8 + 7 * 8 + 8, "AdjointArgs layout");
+
 thread_local std::string g_error;
 
 int32_t fail(int32_t code, const std::string& msg) {
@@ -77,6 +91,7 @@ struct Kernel {
   int32_t block = 64;
   bool persistent = false;
   bool unit_weights = false;
+  bool adjoint = false;
   std::map<int, Loaded> per_device;
 };
 
@@ -184,6 +199,9 @@ int32_t mdx_kernel_create(const void* code, size_t code_size, const char* entry,
   k->block = block;
   k->persistent = (flags & MDX_KERNEL_PERSISTENT) != 0;
   k->unit_weights = (flags & MDX_KERNEL_UNIT_WEIGHTS) != 0;
+  k->adjoint = (flags & MDX_KERNEL_ADJOINT) != 0;
+  if (k->adjoint && k->persistent)
+    return fail(1, "mdx_kernel_create: an adjoint kernel is not a persistent kernel");
   std::lock_guard<std::mutex> lock(g_mutex);
   g_kernels.push_back(std::move(k));
   *handle = (int64_t)g_kernels.size();
@@ -263,6 +281,7 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
     return fail(1, "mdx_solve_launch: this kernel reads error weights; weights is NULL");
   if (hook_size > 0 && !hook_out) return fail(1, "mdx_solve_launch: hook_size > 0 needs hook_out");
   if (max_steps < 0) return fail(1, "mdx_solve_launch: max_steps must be non-negative");
+  if (k->adjoint) return fail(1, "mdx_solve_launch: an adjoint kernel is launched by mdx_adjoint_launch");
   if (k->persistent && (!workspace || workspace_bytes < 8))
     return fail(1, "mdx_solve_launch: a persistent kernel needs a workspace of at least 8 bytes");
   if (n_traj == 0) return 0;
@@ -326,6 +345,85 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
   void* kargs[] = {&a};
   e = hipModuleLaunchKernel(l->fn, (unsigned)blocks, 1, 1, (unsigned)k->block, 1, 1, 0, s, kargs,
                             nullptr);
+  if (e != hipSuccess) return hip_fail(e, ("hipModuleLaunchKernel(" + k->entry + ")").c_str());
+  return 0;
+}
+
+int64_t mdx_adjoint_workspace_bytes(int64_t n_traj, int32_t n_vars, int32_t ckpt_every,
+                                    int32_t max_ckpt) {
+  if (n_traj < 0 || n_vars < 1 || ckpt_every < 1 || max_ckpt < 1) return -1;
+  return n_traj * 8 * ((int64_t)max_ckpt * (n_vars + 8) + (int64_t)ckpt_every * (n_vars + 4));
+}
+
+int32_t mdx_adjoint_launch(int64_t handle, void* stream, int64_t n_traj, int32_t n_vars,
+                           int32_t n_params, int32_t n_save, int32_t layout, const double* y0,
+                           const double* times, const double* params, double dt0, double rtol,
+                           double atol, int32_t max_steps, const double* weights,
+                           const double* gbar, double* grad_y0, double* grad_p, int32_t* status,
+                           int32_t ckpt_every, int32_t max_ckpt, void* workspace,
+                           int64_t workspace_bytes) {
+  Kernel* k = nullptr;
+  if (int32_t rc = lookup(handle, &k)) return rc;
+  if (!k->adjoint) return fail(1, "mdx_adjoint_launch: not an adjoint kernel");
+  if (n_traj < 0 || n_vars < 1 || n_params < 0 || n_save < 1)
+    return fail(1, "mdx_adjoint_launch: bad extents");
+  if (layout != 0 && layout != 1) return fail(1, "mdx_adjoint_launch: layout must be 0 or 1");
+  if (ckpt_every < 1 || max_ckpt < 1)
+    return fail(1, "mdx_adjoint_launch: ckpt_every and max_ckpt must be positive");
+  if (!y0 || !times || (!params && n_params > 0) || !gbar || !grad_y0 ||
+      (!grad_p && n_params > 0) || !status || !workspace)
+    return fail(1, "mdx_adjoint_launch: null buffer");
+  if (!weights && !k->unit_weights)
+    return fail(1, "mdx_adjoint_launch: this kernel reads error weights; weights is NULL");
+  if (max_steps < 0) return fail(1, "mdx_adjoint_launch: max_steps must be non-negative");
+  const int64_t need = mdx_adjoint_workspace_bytes(n_traj, n_vars, ckpt_every, max_ckpt);
+  if (workspace_bytes < need)
+    return fail(1, "mdx_adjoint_launch: workspace holds " + std::to_string(workspace_bytes) +
+                       " bytes, needs " + std::to_string(need));
+  if (n_traj == 0) return 0;
+
+  AdjointArgs g;
+  std::memset(&g, 0, sizeof(g));
+  SolveArgs& a = g.s;
+  a.y0 = y0;
+  a.times = times;
+  a.params = params;
+  a.weights = weights;
+  a.dt0 = dt0;
+  a.rtol = rtol;
+  a.atol = atol;
+  a.n_traj = n_traj;
+  a.max_steps = max_steps;
+  a.n_save = n_save;
+  g.gbar = gbar;
+  g.grad_y0 = grad_y0;
+  g.grad_p = grad_p;
+  g.status = status;
+  g.ws = (double*)workspace;
+  g.ckpt_every = ckpt_every;
+  g.max_ckpt = max_ckpt;
+  if (layout == 0) {
+    a.y0_ts = n_vars, a.y0_vs = 1;
+    a.p_ts = n_params, a.p_vs = 1;
+    a.w_ts = n_vars, a.w_vs = 1;
+    g.g_ts = (int64_t)n_save * n_vars, g.g_ss = n_vars, g.g_vs = 1;
+    g.gy_ts = n_vars, g.gy_vs = 1;
+    g.gp_ts = n_params, g.gp_vs = 1;
+  } else {
+    a.y0_ts = 1, a.y0_vs = n_traj;
+    a.p_ts = 1, a.p_vs = n_traj;
+    a.w_ts = 1, a.w_vs = n_traj;
+    g.g_ts = 1, g.g_ss = (int64_t)n_vars * n_traj, g.g_vs = n_traj;
+    g.gy_ts = 1, g.gy_vs = n_traj;
+    g.gp_ts = 1, g.gp_vs = n_traj;
+  }
+  Loaded* l = nullptr;
+  if (int32_t rc = loaded_on_current_device(k, &l)) return rc;
+  const int64_t blocks = (n_traj + k->block - 1) / k->block;
+  if (blocks > 0x7fffffff) return fail(1, "mdx_adjoint_launch: grid too large");
+  void* kargs[] = {&g};
+  hipError_t e = hipModuleLaunchKernel(l->fn, (unsigned)blocks, 1, 1, (unsigned)k->block, 1, 1, 0,
+                                       (hipStream_t)stream, kargs, nullptr);
   if (e != hipSuccess) return hip_fail(e, ("hipModuleLaunchKernel(" + k->entry + ")").c_str());
   return 0;
 }

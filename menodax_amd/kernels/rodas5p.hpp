@@ -90,6 +90,9 @@ __device__ constexpr double H3[8] = {11.419903575922262, 2.8879645146136994, 72.
 #ifndef MDX_ROLLED
 #define MDX_ROLLED 0
 #endif
+#ifndef MDX_ADJOINT
+#define MDX_ADJOINT 0
+#endif
 // The rolled kernel copies the parameter row into per-lane scratch: each read
 // is then one load at a constant offset, with no address kept in registers.
 #if MDX_PARAMS_IN_REGS || MDX_ROLLED
@@ -329,6 +332,34 @@ __device__ __forceinline__ void dense_solve(const T* __restrict__ lu, const int*
   }
 }
 
+// M^T x = b for the factorisation above (PM = LU, swaps in ipiv): U^T z = b,
+// L^T w = z, then the swaps undone in reverse order. Used by the adjoint.
+template <class T>
+__device__ __forceinline__ void dense_solve_t(const T* __restrict__ lu, const int* __restrict__ ipiv,
+                                              T* __restrict__ r) {
+#pragma unroll 1
+  for (int i = 0; i < N; ++i) {
+    T acc = r[i];
+    for (int j = 0; j < i; ++j) acc = fmaT(-lu[j * N + i], r[j], acc);
+    r[i] = acc / lu[i * N + i];
+  }
+#pragma unroll 1
+  for (int i = N - 1; i >= 0; --i) {
+    T acc = r[i];
+    for (int j = i + 1; j < N; ++j) acc = fmaT(-lu[j * N + i], r[j], acc);
+    r[i] = acc;
+  }
+#pragma unroll 1
+  for (int i = N - 1; i >= 0; --i) {
+    const int piv = ipiv[i];
+    if (piv != i) {
+      const T a = r[i];
+      r[i] = r[piv];
+      r[piv] = a;
+    }
+  }
+}
+
 __device__ __forceinline__ int diag_slot(int d) {
 #if MDX_SPARSE
   return sys::DIAG[d];
@@ -355,6 +386,19 @@ __device__ __forceinline__ void solve(const LUT* __restrict__ lu, const int* __r
   dense_solve(lu, ipiv, r);
 #endif
 }
+
+#if MDX_ADJOINT
+// M^T x = b against the step's factorisation, in double (the adjoint).
+__device__ __forceinline__ void solve_t(const double* __restrict__ lu, const int* __restrict__ ipiv,
+                                        double* __restrict__ r) {
+#if MDX_SPARSE
+  (void)ipiv;
+  sys::sparse_solve_t(lu, r);
+#else
+  dense_solve_t(lu, ipiv, r);
+#endif
+}
+#endif
 
 // The joint right-hand side at z: the state block, then each direction's
 // J_y s + J_p e (modax/_sensitivity.py:198-243).
@@ -474,8 +518,10 @@ __device__ __forceinline__ void init(State& s, const SolveArgs& a, i64 i, const 
   const double tf = times[a.n_save - 1];
 #pragma unroll
   for (int j = 0; j < NA; ++j) YJ(s, j) = a.y0[i * a.y0_ts + (i64)j * a.y0_vs];
+#if !MDX_ADJOINT
 #pragma unroll
   for (int j = 0; j < NA; ++j) a.hist[i * a.h_ts + (i64)j * a.h_vs] = YJ(s, j);
+#endif
   s.t = times[0];
   s.dt = a.dt0 > 0.0 ? a.dt0 : (tf - times[0]) * 1e-6;
 #if MDX_TF_INDEX >= 0
@@ -693,6 +739,7 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
       const double ts = times[s.save_idx];
       const double at = absT(ts);
       if (!(ts <= t_new + 1e-12 * (1.0 > at ? 1.0 : at))) break;
+#if !MDX_ADJOINT
       const double theta = (ts - t) / dt_use;
       const double theta1 = 1.0 - theta;
       double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
@@ -715,6 +762,7 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
         v = fmaT(theta1, v, y_new);
         *row = fmaT(theta, v, theta1 * YJ(s, j));
       }
+#endif
       s.save_idx++;
     }
 #pragma unroll
@@ -744,6 +792,7 @@ __device__ __forceinline__ void finish(State& s, const SolveArgs& a, i64 i) {
 }  // namespace r5
 }  // namespace mdx
 
+#if !MDX_ADJOINT
 // A register cap, when the caller set one (the reference's max_registers).
 #if defined(MDX_MAX_VGPRS) && MDX_MAX_VGPRS > 0
 #define MDX_REG_CAP __attribute__((amdgpu_num_vgpr(MDX_MAX_VGPRS > 256 ? 256 : MDX_MAX_VGPRS)))
@@ -798,3 +847,4 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p_
     }
   }
 }
+#endif  // !MDX_ADJOINT

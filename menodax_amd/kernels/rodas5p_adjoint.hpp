@@ -1,0 +1,315 @@
+// rodas5p_adjoint.hpp -- discrete adjoint of the Rodas5P ensemble solve, one
+// trajectory per lane.
+//
+// Included after rodas5p.hpp in a source generated with MDX_ADJOINT=1 (the
+// forward step then writes no history) and MDX_ROLLED=1, whose sys namespace
+// also holds f_vjp / jac_vjp / dt_vjp (reverse-mode contractions of the
+// callback) and, for a sparse system, sparse_solve_t.
+//
+// Given dL/dhist for a loss of the saved history, each lane returns dL/dy0 and
+// dL/dparams of its trajectory, differentiating the solver's own arithmetic:
+// the accepted steps with their step sizes held fixed (the step-size
+// controller is not differentiated), every stage, the iteration matrix's
+// dependence on the state and parameters through J(y, p), the df/dt term, and
+// the dense output that produced each save.
+//
+//   phase A  the forward solve again, storing the whole step state (state
+//            vector, time, step size, controller memory, counters) before
+//            every ckpt_every-th accepted step;
+//   phase B  checkpoints last to first: replay the segment from its
+//            checkpoint -- the same code, so the same bits -- recording each
+//            accepted step's (y_n, t_n, h_n, saves it produced), then run back
+//            through the recorded steps.
+//
+// Reverse of one step (y_{n+1} = u7 + k7, k_s = M^{-1} r_s, M = I/(h gamma) - J):
+//   r_bar_s  = M^{-T} k_bar_s                    (s = 7 .. 0)
+//   J_bar   += r_bar_s k_s^T   -> d/d(y,p) of r_bar_s . J(y, p) k_s
+//   dT_bar  += h d_s r_bar_s ;  k_bar_q += C_sq / h r_bar_s
+//   u_bar_s += J(u_s)^T r_bar_s ; p_bar += J_p(u_s)^T r_bar_s
+//   u_bar_s flows to y (and to k_q through the A rows, or to u_{s-1}, k_{s-1}
+//   for the two end stages)
+// Workspace, per lane, [slot][lane] (coalesced): max_ckpt checkpoints of
+// N + 8 doubles, then ckpt_every step records of N + 4 doubles.
+#pragma once
+
+namespace mdx {
+namespace r5 {
+
+struct AdjointArgs {
+  SolveArgs s;             // the forward solve's inputs; hist/counters unused
+  const double* gbar;      // dL/dhist, strides g_ts, g_ss, g_vs
+  double* grad_y0;         // (traj, var), strides gy_ts, gy_vs
+  double* grad_p;          // (traj, param), strides gp_ts, gp_vs
+  int* status;             // (traj,): 0 ok, 1 checkpoint space exhausted
+  double* ws;              // workspace
+  i64 g_ts, g_ss, g_vs;
+  i64 gy_ts, gy_vs, gp_ts, gp_vs;
+  int ckpt_every, max_ckpt;
+};
+
+constexpr int CK = N + 8;   // checkpoint words
+constexpr int RC = N + 4;   // step record words
+constexpr int NPB = MDX_NP > 0 ? MDX_NP : 1;
+
+__device__ __forceinline__ double& wsat(const AdjointArgs& g, i64 i, i64 slot) {
+  return g.ws[slot * g.s.n_traj + i];
+}
+
+__device__ __forceinline__ void store_ckpt(const AdjointArgs& g, i64 i, int c, const State& s) {
+  const i64 b = (i64)c * CK;
+  for (int j = 0; j < N; ++j) wsat(g, i, b + j) = s.y[j];
+  wsat(g, i, b + N + 0) = s.t;
+  wsat(g, i, b + N + 1) = s.dt;
+  wsat(g, i, b + N + 2) = s.err_prev;
+  wsat(g, i, b + N + 3) = s.err_prev2;
+  wsat(g, i, b + N + 4) = (double)s.save_idx;
+  wsat(g, i, b + N + 5) = (double)s.n_steps;
+  wsat(g, i, b + N + 6) = (double)s.accepted;
+  wsat(g, i, b + N + 7) = (double)s.rejected;
+}
+
+__device__ __forceinline__ void load_ckpt(const AdjointArgs& g, i64 i, int c, State& s) {
+  const i64 b = (i64)c * CK;
+  for (int j = 0; j < N; ++j) s.y[j] = wsat(g, i, b + j);
+  s.t = wsat(g, i, b + N + 0);
+  s.dt = wsat(g, i, b + N + 1);
+  s.err_prev = wsat(g, i, b + N + 2);
+  s.err_prev2 = wsat(g, i, b + N + 3);
+  s.save_idx = (int)wsat(g, i, b + N + 4);
+  s.n_steps = (int)wsat(g, i, b + N + 5);
+  s.accepted = (int)wsat(g, i, b + N + 6);
+  s.rejected = (int)wsat(g, i, b + N + 7);
+}
+
+__device__ __forceinline__ double gbar_at(const AdjointArgs& g, i64 i, int sv, int j) {
+  return g.gbar[i * g.g_ts + (i64)sv * g.g_ss + (i64)j * g.g_vs];
+}
+
+// The step size the forward step takes from state s (its first lines).
+__device__ __forceinline__ double step_size(const State& s) {
+  double dt_use = s.dt;
+  if (dt_use > s.tf - s.t) dt_use = s.tf - s.t;
+  if (dt_use < 1e-30) dt_use = 1e-30;
+  return dt_use;
+}
+
+// The step's factorisation and stage vectors at (y, t, h), by the forward
+// step's arithmetic.
+__device__ void recompute(const double* __restrict__ y, double t, double h, PA& p,
+                          LUT* __restrict__ lu, int* __restrict__ ipiv, LUT (*__restrict__ k)[N]) {
+  const double inv_dt = 1.0 / h;
+  const double t_end = t + h;
+  const double dtgamma_inv = 1.0 / (h * GAMMA);
+#if MDX_NEEDS_CLEAR
+  for (int q = 0; q < MDX_NLU; ++q) lu[q] = (LUT)0;
+#endif
+  double dT[N];
+  sys::neg_jac(y, t, p, lu, dT);
+  for (int d = 0; d < N; ++d) lu[diag_slot(d)] = lu[diag_slot(d)] + (LUT)dtgamma_inv;
+  factor(lu, ipiv);
+  double u[N];
+#pragma unroll 1
+  for (int S = 0; S < 8; ++S) {
+    double ts;
+    if (S == 0) {
+      for (int j = 0; j < N; ++j) u[j] = y[j];
+      ts = t;
+    } else if (S < 6) {
+      for (int j = 0; j < N; ++j) {
+        double acc = AM[S - 1][0] * (double)k[0][j];
+        for (int q = 1; q < S; ++q) acc = fmaT(AM[S - 1][q], (double)k[q][j], acc);
+        u[j] = y[j] + acc;
+      }
+      ts = S < 5 ? t + CT[S - 1] * h : t_end;
+    } else {
+      for (int j = 0; j < N; ++j) u[j] = u[j] + (double)k[S - 1][j];
+      ts = t_end;
+    }
+    double work[N];
+    sys::rhs(u, ts, p, work);
+    LUT r[N];
+    for (int j = 0; j < N; ++j) {
+      double v;
+      if (S == 0) {
+        v = fmaT(h * DV[0], DTJ(j), work[j]);
+      } else {
+        double acc = CM[S][0] * (double)k[0][j];
+        for (int q = 1; q < S; ++q) acc = fmaT(CM[S][q], (double)k[q][j], acc);
+        v = S < 5 ? fmaT(acc, inv_dt, fmaT(h * DV[S], DTJ(j), work[j])) : fmaT(acc, inv_dt, work[j]);
+      }
+      r[j] = (LUT)v;
+    }
+    solve(lu, ipiv, r);
+    for (int j = 0; j < N; ++j) k[S][j] = r[j];
+  }
+}
+
+// u_S of the step (the argument of stage S's right-hand side) and its time.
+__device__ __forceinline__ double stage_point(int S, const double* __restrict__ y, double t,
+                                              double h, LUT (*__restrict__ k)[N],
+                                              double* __restrict__ u) {
+  if (S == 0) {
+    for (int j = 0; j < N; ++j) u[j] = y[j];
+    return t;
+  }
+  const int A = S < 6 ? S - 1 : 4;
+  for (int j = 0; j < N; ++j) {
+    double acc = AM[A][0] * (double)k[0][j];
+    for (int q = 1; q <= A; ++q) acc = fmaT(AM[A][q], (double)k[q][j], acc);
+    u[j] = y[j] + acc;
+  }
+  if (S >= 6) for (int j = 0; j < N; ++j) u[j] = u[j] + (double)k[5][j];
+  if (S >= 7) for (int j = 0; j < N; ++j) u[j] = u[j] + (double)k[6][j];
+  return S == 0 ? t : (S < 5 ? t + CT[S - 1] * h : t + h);
+}
+
+// lam: dL/dy_{n+1} in, dL/dy_n out; pbar accumulates dL/dp.
+__device__ void reverse_step(const AdjointArgs& g, i64 i, PA& p, const double* __restrict__ y,
+                             double t, double h, int save_lo, int save_hi,
+                             double* __restrict__ lam, double* __restrict__ pbar) {
+  const SolveArgs& a = g.s;
+  LUT lu[MDX_NLU];
+  int ipiv[N];
+  LUT k[8][N];
+  recompute(y, t, h, p, lu, ipiv, k);
+  double lud[MDX_NLU];
+  for (int q = 0; q < MDX_NLU; ++q) lud[q] = (double)lu[q];
+  const double inv_dt = 1.0 / h;
+
+  double kbar[8][N], ybar[N], ucarry[N], dTbar[N];
+  for (int q = 0; q < 8; ++q)
+    for (int j = 0; j < N; ++j) kbar[q][j] = 0.0;
+  for (int j = 0; j < N; ++j) {
+    ybar[j] = 0.0;
+    dTbar[j] = 0.0;
+    ucarry[j] = lam[j];   // y_{n+1} = u7 + k7
+    kbar[7][j] = lam[j];
+  }
+  // saves the step produced: theta1 y + theta (y_new + theta1 (h1 + theta (h2 + theta h3)))
+  for (int sv = save_lo; sv < save_hi; ++sv) {
+    const double theta = (a.times[sv] - t) / h;
+    const double theta1 = 1.0 - theta;
+    const double c1 = theta * theta1, c2 = c1 * theta, c3 = c2 * theta;
+    for (int j = 0; j < N; ++j) {
+      const double gj = gbar_at(g, i, sv, j);
+      ybar[j] += theta1 * gj;
+      ucarry[j] += theta * gj;
+      kbar[7][j] += theta * gj;
+      for (int q = 0; q < 8; ++q) kbar[q][j] += (H1[q] * c1 + H2[q] * c2 + H3[q] * c3) * gj;
+    }
+  }
+#pragma unroll 1
+  for (int S = 7; S >= 0; --S) {
+    double rb[N], kd[N], u[N], ub[N];
+    for (int j = 0; j < N; ++j) {
+      rb[j] = kbar[S][j];
+      kd[j] = (double)k[S][j];
+      ub[j] = 0.0;
+    }
+    solve_t(lud, ipiv, rb);
+    sys::jac_vjp(y, t, p, rb, kd, ybar, pbar);
+    if (S < 5) {
+      const double dd = h * DV[S];
+      for (int j = 0; j < N; ++j) dTbar[j] += dd * rb[j];
+    }
+    for (int q = 0; q < S; ++q) {
+      const double c = CM[S][q] * inv_dt;
+      for (int j = 0; j < N; ++j) kbar[q][j] += c * rb[j];
+    }
+    const double ts = stage_point(S, y, t, h, k, u);
+    sys::f_vjp(u, ts, p, rb, ub, pbar);
+    if (S >= 5) {
+      for (int j = 0; j < N; ++j) ub[j] += ucarry[j];
+    }
+    if (S == 7 || S == 6) {
+      // u_S = u_{S-1} + k_{S-1}
+      for (int j = 0; j < N; ++j) {
+        ucarry[j] = ub[j];
+        kbar[S - 1][j] += ub[j];
+      }
+    } else if (S > 0) {
+      // u_S = y + sum_q A_{S,q} k_q
+      for (int j = 0; j < N; ++j) ybar[j] += ub[j];
+      for (int q = 0; q < S; ++q)
+        for (int j = 0; j < N; ++j) kbar[q][j] += AM[S - 1][q] * ub[j];
+    } else {
+      for (int j = 0; j < N; ++j) ybar[j] += ub[j];
+    }
+  }
+#if !MDX_DT_ZERO
+  sys::dt_vjp(y, t, p, dTbar, ybar, pbar);
+#endif
+  for (int j = 0; j < N; ++j) lam[j] = ybar[j];
+}
+
+}  // namespace r5
+}  // namespace mdx
+
+extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_rodas5p_adjoint(
+    mdx::r5::AdjointArgs g) {
+  using namespace mdx::r5;
+  const mdx::SolveArgs& a = g.s;
+  const mdx::i64 i = (mdx::i64)blockIdx.x * MDX_BLOCK + threadIdx.x;
+  if (i >= a.n_traj) return;
+  PA p;
+  p.load(a, i);
+  State s;
+  init(s, a, i, p);
+  const int C = g.ckpt_every;
+
+  // phase A: the forward solve, checkpointing before every C-th accepted step
+  int n_ck = 0, last = -1, status = 0;
+  while (active(s, a)) {
+    if (s.accepted % C == 0 && s.accepted != last) {
+      if (n_ck < g.max_ckpt) store_ckpt(g, i, n_ck, s);
+      else status = 1;
+      ++n_ck;
+      last = s.accepted;
+    }
+    step(s, p, a, i);
+  }
+  if (status != 0) n_ck = 0;  // not enough room: report, leave the gradients NaN
+
+  // saves the loop never reached hold the final state
+  double lam[N], pbar[NPB];
+  for (int j = 0; j < N; ++j) lam[j] = 0.0;
+  for (int m = 0; m < NPB; ++m) pbar[m] = 0.0;
+  for (int sv = s.save_idx; sv < a.n_save; ++sv)
+    for (int j = 0; j < N; ++j) lam[j] += gbar_at(g, i, sv, j);
+
+  // phase B: segments last to first
+  const mdx::i64 rec0 = (mdx::i64)g.max_ckpt * CK;
+#pragma unroll 1
+  for (int c = n_ck - 1; c >= 0; --c) {
+    load_ckpt(g, i, c, s);
+    const int target = s.accepted + C;
+    int n_rec = 0;
+    while (active(s, a) && s.accepted < target) {
+      const mdx::i64 b = rec0 + (mdx::i64)n_rec * RC;
+      for (int j = 0; j < N; ++j) wsat(g, i, b + j) = s.y[j];
+      wsat(g, i, b + N + 0) = s.t;
+      wsat(g, i, b + N + 1) = step_size(s);
+      wsat(g, i, b + N + 2) = (double)s.save_idx;
+      const int acc0 = s.accepted;
+      step(s, p, a, i);
+      if (s.accepted > acc0) {
+        wsat(g, i, b + N + 3) = (double)s.save_idx;
+        ++n_rec;
+      }
+    }
+#pragma unroll 1
+    for (int r = n_rec - 1; r >= 0; --r) {
+      const mdx::i64 b = rec0 + (mdx::i64)r * RC;
+      double y[N];
+      for (int j = 0; j < N; ++j) y[j] = wsat(g, i, b + j);
+      reverse_step(g, i, p, y, wsat(g, i, b + N + 0), wsat(g, i, b + N + 1),
+                   (int)wsat(g, i, b + N + 2), (int)wsat(g, i, b + N + 3), lam, pbar);
+    }
+  }
+  // hist[0] is y0 itself
+  const double bad = status != 0 ? __builtin_nan("") : 0.0;
+  for (int j = 0; j < N; ++j)
+    g.grad_y0[i * g.gy_ts + (mdx::i64)j * g.gy_vs] = lam[j] + gbar_at(g, i, 0, j) + bad;
+  for (int m = 0; m < MDX_NP; ++m) g.grad_p[i * g.gp_ts + (mdx::i64)m * g.gp_vs] = pbar[m] + bad;
+  g.status[i] = status;
+}

@@ -236,6 +236,10 @@ def test_the_emitted_device_code_solves_the_matrix(mask, ordering):
     r = b.copy()
     _run_emitted(src, "sparse_solve", lu, r)
     np.testing.assert_allclose(r, np.linalg.solve(dense, b), rtol=1e-9)
+    # the transposed solve the adjoint uses, with the same factors
+    r = b.copy()
+    _run_emitted(src, "sparse_solve_t", lu, r)
+    np.testing.assert_allclose(r, np.linalg.solve(dense.T, b), rtol=1e-9)
 
 
 def test_table_driven_source_is_emitted_above_the_thresholds(monkeypatch):
@@ -244,6 +248,7 @@ def test_table_driven_source_is_emitted_above_the_thresholds(monkeypatch):
     solver = SD.SparseDirectSolver(*_layout_and_map(_ring(6)))
     src = solver.device_source()
     assert "__constant__ int FACTOR_TAB" in src and "__constant__ int SOLVE_TAB" in src
+    assert "__constant__ int SOLVE_T_TAB" in src
 
 
 def _layout_and_map(mask):
