@@ -196,6 +196,20 @@ struct LdsVec {
 // In float storage the reciprocal and multiplier are double and each update is
 // one double fma rounded to float, as the reference's typing has it.
 // ---------------------------------------------------------------------------
+// What the dense factorisation leaves beside the matrix: the row swaps, and
+// for each pivot u_ii the double reciprocal 1/u_ii the elimination formed
+// anyway. With a float matrix the back substitution multiplies by it instead
+// of dividing: for float a and u with a normal float quotient,
+// float(double(a) * (1/u)) is the correctly rounded a/u -- the double product
+// is within 2^-52 of a/u relative, while a quotient of two 24-bit significands
+// that is not exactly a float midpoint lies at least 2^-49 from every midpoint
+// (and an exact midpoint of the normal range needs u a power of two, where the
+// product is exact), so both round to the same float.
+struct Pivots {
+  int p[N];
+  double rd[N];
+};
+
 // Up to this size the dense LU is unrolled with every index a constant, so the
 // matrix stays in registers; above it the loops are kept and the matrix lives
 // in scratch (a dense solve at that size is the uninformative end of the
@@ -205,8 +219,60 @@ struct LdsVec {
 #endif
 constexpr bool DENSE_UNROLLED = N <= MDX_DENSE_UNROLL_MAX;
 
+// U x = z, the back substitution of the dense solve. With double storage each
+// row divides by its pivot. With float storage each row multiplies by the
+// pivot's double reciprocal (see Pivots); a quotient in the float subnormal
+// range can sit exactly on a midpoint of that coarser grid with u not a power
+// of two, so when any row's result is nonzero and below FLT_MIN the whole
+// substitution is redone with divisions -- rare, and one branch per solve.
+template <bool UNROLLED>
+__device__ __forceinline__ void back_substitute(const double* __restrict__ lu, const Pivots& ipiv,
+                                                double* __restrict__ r) {
+  (void)ipiv;
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    double acc = r[i];
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
+    r[i] = acc / lu[i * N + i];
+  }
+}
+
+template <bool UNROLLED>
+__device__ __forceinline__ void back_substitute(const float* __restrict__ lu, const Pivots& ipiv,
+                                                float* __restrict__ r) {
+  float z[N];
+  bool tiny = false;
+#pragma unroll
+  for (int i = 0; i < N; ++i) z[i] = r[i];
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    float acc = r[i];
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
+    const float q = (float)((double)acc * ipiv.rd[i]);
+    tiny |= __builtin_fabsf(q) < 1.17549435e-38f && acc != 0.0f;
+    r[i] = q;
+  }
+  if (tiny) {
+    // the first row's divisor passes through an opaque move so the divisions
+    // stay on this branch instead of being computed on every solve
+    float u0 = lu[0];
+    asm volatile("; subnormal quotient" : "+v"(u0));
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+      float acc = z[i];
+#pragma unroll
+      for (int j = i + 1; j < N; ++j) acc = fmaT(-lu[i * N + j], z[j], acc);
+      z[i] = acc / (i == 0 ? u0 : lu[i * N + i]);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] = z[i];
+  }
+}
+
 template <class T>
-__device__ __forceinline__ void dense_factor(T* __restrict__ lu, int* __restrict__ ipiv) {
+__device__ __forceinline__ void dense_factor(T* __restrict__ lu, Pivots& ipiv) {
   if constexpr (DENSE_UNROLLED) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -220,7 +286,7 @@ __device__ __forceinline__ void dense_factor(T* __restrict__ lu, int* __restrict
           piv = k;
         }
       }
-      ipiv[i] = piv;
+      ipiv.p[i] = piv;
 #pragma unroll
       for (int k = i + 1; k < N; ++k) {
         const bool sw = (piv == k);
@@ -232,8 +298,9 @@ __device__ __forceinline__ void dense_factor(T* __restrict__ lu, int* __restrict
         }
       }
       const T pv = lu[i * N + i];
+      const double inv = 1.0 / (double)pv;
+      ipiv.rd[i] = inv;
       if (pv != (T)0) {
-        const double inv = 1.0 / (double)pv;
 #pragma unroll
         for (int k = i + 1; k < N; ++k) {
           const double f = (double)lu[k * N + i] * inv;
@@ -257,7 +324,7 @@ __device__ __forceinline__ void dense_factor(T* __restrict__ lu, int* __restrict
           piv = k;
         }
       }
-      ipiv[i] = piv;
+      ipiv.p[i] = piv;
       if (piv != i) {
 #pragma unroll 1
         for (int j = 0; j < N; ++j) {
@@ -267,8 +334,9 @@ __device__ __forceinline__ void dense_factor(T* __restrict__ lu, int* __restrict
         }
       }
       const T pv = lu[i * N + i];
+      const double inv = 1.0 / (double)pv;
+      ipiv.rd[i] = inv;
       if (pv != (T)0) {
-        const double inv = 1.0 / (double)pv;
 #pragma unroll 1
         for (int k = i + 1; k < N; ++k) {
           const double f = (double)lu[k * N + i] * inv;
@@ -283,12 +351,12 @@ __device__ __forceinline__ void dense_factor(T* __restrict__ lu, int* __restrict
 }
 
 template <class T>
-__device__ __forceinline__ void dense_solve(const T* __restrict__ lu, const int* __restrict__ ipiv,
+__device__ __forceinline__ void dense_solve(const T* __restrict__ lu, const Pivots& ipiv,
                                             T* __restrict__ r) {
   if constexpr (DENSE_UNROLLED) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      const int piv = ipiv[i];
+      const int piv = ipiv.p[i];
 #pragma unroll
       for (int k = i + 1; k < N; ++k) {
         const bool sw = (piv == k);
@@ -301,17 +369,11 @@ __device__ __forceinline__ void dense_solve(const T* __restrict__ lu, const int*
       for (int j = 0; j < i; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
       r[i] = acc;
     }
-#pragma unroll
-    for (int i = N - 1; i >= 0; --i) {
-      T acc = r[i];
-#pragma unroll
-      for (int j = i + 1; j < N; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
-      r[i] = acc / lu[i * N + i];
-    }
+    back_substitute<true>(lu, ipiv, r);
   } else {
 #pragma unroll 1
     for (int i = 0; i < N; ++i) {
-      const int piv = ipiv[i];
+      const int piv = ipiv.p[i];
       if (piv != i) {
         const T a = r[i];
         r[i] = r[piv];
@@ -335,7 +397,7 @@ __device__ __forceinline__ void dense_solve(const T* __restrict__ lu, const int*
 // M^T x = b for the factorisation above (PM = LU, swaps in ipiv): U^T z = b,
 // L^T w = z, then the swaps undone in reverse order. Used by the adjoint.
 template <class T>
-__device__ __forceinline__ void dense_solve_t(const T* __restrict__ lu, const int* __restrict__ ipiv,
+__device__ __forceinline__ void dense_solve_t(const T* __restrict__ lu, const Pivots& ipiv,
                                               T* __restrict__ r) {
 #pragma unroll 1
   for (int i = 0; i < N; ++i) {
@@ -351,7 +413,7 @@ __device__ __forceinline__ void dense_solve_t(const T* __restrict__ lu, const in
   }
 #pragma unroll 1
   for (int i = N - 1; i >= 0; --i) {
-    const int piv = ipiv[i];
+    const int piv = ipiv.p[i];
     if (piv != i) {
       const T a = r[i];
       r[i] = r[piv];
@@ -368,7 +430,7 @@ __device__ __forceinline__ int diag_slot(int d) {
 #endif
 }
 
-__device__ __forceinline__ void factor(LUT* __restrict__ lu, int* __restrict__ ipiv) {
+__device__ __forceinline__ void factor(LUT* __restrict__ lu, Pivots& ipiv) {
 #if MDX_SPARSE
   (void)ipiv;
   sys::sparse_factor(lu);
@@ -377,7 +439,7 @@ __device__ __forceinline__ void factor(LUT* __restrict__ lu, int* __restrict__ i
 #endif
 }
 
-__device__ __forceinline__ void solve(const LUT* __restrict__ lu, const int* __restrict__ ipiv,
+__device__ __forceinline__ void solve(const LUT* __restrict__ lu, const Pivots& ipiv,
                                       LUT* __restrict__ r) {
 #if MDX_SPARSE
   (void)ipiv;
@@ -389,7 +451,7 @@ __device__ __forceinline__ void solve(const LUT* __restrict__ lu, const int* __r
 
 #if MDX_ADJOINT
 // M^T x = b against the step's factorisation, in double (the adjoint).
-__device__ __forceinline__ void solve_t(const double* __restrict__ lu, const int* __restrict__ ipiv,
+__device__ __forceinline__ void solve_t(const double* __restrict__ lu, const Pivots& ipiv,
                                         double* __restrict__ r) {
 #if MDX_SPARSE
   (void)ipiv;
@@ -436,7 +498,7 @@ template <class Z, class KS>
 __device__ __forceinline__ void sens_stage(int S, const Z& ybase, double t, const PA& p,
                                            const double* __restrict__ sv,
                                            const LUT* __restrict__ lu,
-                                           const int* __restrict__ ipiv, KS ks) {
+                                           const Pivots& ipiv, KS ks) {
 #if MDX_NSENS > 0
   double ky[N];
 #pragma unroll
@@ -463,7 +525,7 @@ __device__ __forceinline__ void sens_stage(int S, const Z& ybase, double t, cons
 template <int S>
 __device__ __forceinline__ void stage(const double* __restrict__ work,
                                       const double* __restrict__ dT, LUT (*__restrict__ k)[NA],
-                                      const LUT* __restrict__ lu, const int* __restrict__ ipiv,
+                                      const LUT* __restrict__ lu, const Pivots& ipiv,
                                       double dt_use, double inv_dt, double* __restrict__ sv) {
   LUT r[N];
 #pragma unroll
@@ -572,7 +634,7 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
   sens_time_derivative(ybase, t, p, dT);
 #pragma unroll
   for (int d = 0; d < N; ++d) lu[diag_slot(d)] = lu[diag_slot(d)] + (LUT)dtgamma_inv;
-  int ipiv[N];
+  Pivots ipiv;
   factor(lu, ipiv);
 
 #if !MDX_LDS_K
@@ -657,20 +719,29 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
     rhs_aug(u, ts, p, work);
 #endif
     // rhs_j = LU(work_j + dt*d_S*dT_j + (sum_q C_Sq k_qj) / dt) (rodas5P.py:611-647)
+    // one branch per stage, outside the element loop, so each element is one
+    // formula rather than all three and a select
     LUT r[N];
+    if (S == 0) {
+      const double dd = dt_use * DV[0];
 #pragma unroll
-    for (int j = 0; j < NA; ++j) {
-      double v;
-      if (S == 0)
-        v = fmaT(dt_use * DV[0], DTJ(j), work[j]);
-      else if (S < 5)
-        v = fmaT(acc[j], inv_dt, fmaT(dt_use * DV[S], DTJ(j), work[j]));
-      else
-        v = fmaT(acc[j], inv_dt, work[j]);
-      if (j < N)
-        r[j] = (LUT)v;
-      else
-        sv[j - N] = v;
+      for (int j = 0; j < NA; ++j) {
+        const double v = fmaT(dd, DTJ(j), work[j]);
+        if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
+      }
+    } else if (S < 5) {
+      const double dd = dt_use * DV[S];
+#pragma unroll
+      for (int j = 0; j < NA; ++j) {
+        const double v = fmaT(acc[j], inv_dt, fmaT(dd, DTJ(j), work[j]));
+        if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NA; ++j) {
+        const double v = fmaT(acc[j], inv_dt, work[j]);
+        if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
+      }
     }
     solve(lu, ipiv, r);
 #pragma unroll

@@ -96,7 +96,7 @@ __device__ __forceinline__ double step_size(const State& s) {
 // The step's factorisation and stage vectors at (y, t, h), by the forward
 // step's arithmetic.
 __device__ void recompute(const double* __restrict__ y, double t, double h, PA& p,
-                          LUT* __restrict__ lu, int* __restrict__ ipiv, LUT (*__restrict__ k)[N]) {
+                          LUT* __restrict__ lu, Pivots& ipiv, LUT (*__restrict__ k)[N]) {
   const double inv_dt = 1.0 / h;
   const double t_end = t + h;
   const double dtgamma_inv = 1.0 / (h * GAMMA);
@@ -169,7 +169,7 @@ __device__ void reverse_step(const AdjointArgs& g, i64 i, PA& p, const double* _
                              double* __restrict__ lam, double* __restrict__ pbar) {
   const SolveArgs& a = g.s;
   LUT lu[MDX_NLU];
-  int ipiv[N];
+  Pivots ipiv;
   LUT k[8][N];
   recompute(y, t, h, p, lu, ipiv, k);
   double lud[MDX_NLU];

@@ -114,7 +114,7 @@ def test_grad_matches_central_differences(method):
         kw["lu_precision"] = "fp64"
     dev = torch.device("cuda")
     y0 = torch.tensor(Y0, device=dev, requires_grad=True)
-    p = torch.tensor([A, B], device=dev, requires_grad=True)
+    p = torch.tensor([A, B], dtype=torch.float64, device=dev, requires_grad=True)
     loss = (mod.solve(S.cascade, y0, T_SPAN, p, **kw) ** 2).sum()
     loss.backward()
 
@@ -139,12 +139,12 @@ def test_ensemble_gradients_are_per_trajectory_and_shared_params_sum():
     y0 = np.array([[2.0, 0.5], [1.0, 1.5], [0.3, 0.2]])
     kw = dict(rtol=1e-10, atol=1e-12, lu_precision="fp64")
     yt = torch.tensor(y0, device=dev, requires_grad=True)
-    shared = torch.tensor([A, B], device=dev, requires_grad=True)
+    shared = torch.tensor([A, B], dtype=torch.float64, device=dev, requires_grad=True)
     (rodas5P.solve(S.cascade, yt, T_SPAN, shared, **kw) ** 2).sum().backward()
     total = torch.zeros(2, dtype=torch.float64, device=dev)
     for r in range(3):
         yr = torch.tensor(y0[r], device=dev, requires_grad=True)
-        pr = torch.tensor([A, B], device=dev, requires_grad=True)
+        pr = torch.tensor([A, B], dtype=torch.float64, device=dev, requires_grad=True)
         (rodas5P.solve(S.cascade, yr, T_SPAN, pr, **kw) ** 2).sum().backward()
         torch.testing.assert_close(yt.grad[r], yr.grad, rtol=1e-12, atol=0)
         total += pr.grad
@@ -173,7 +173,7 @@ def test_forward_mode_jvp_contracts_the_sensitivities():
 def test_stats_stay_available_under_grad():
     import torch
 
-    p = torch.tensor([A, B], device="cuda", requires_grad=True)
+    p = torch.tensor([A, B], dtype=torch.float64, device="cuda", requires_grad=True)
     hist, stats = rodas5P.solve(S.cascade, Y0, T_SPAN, p, rtol=1e-8, atol=1e-10,
                                 return_stats=True)
     assert hist.requires_grad and not stats["accepted_steps"].requires_grad
