@@ -50,6 +50,7 @@ void mdx_free(void* p);
 #define MDX_KERNEL_PERSISTENT 1  /* refill kernel; module also holds mdx_reset_queue */
 #define MDX_KERNEL_UNIT_WEIGHTS 2 /* compiled with unit error weights: `weights` unread */
 #define MDX_KERNEL_ADJOINT 4      /* a discrete-adjoint kernel, launched by mdx_adjoint_launch */
+#define MDX_KERNEL_HOOK 8         /* runs a save hook: hook_out (N, hook_size) required */
 
 /* Register a kernel from a code object: `entry` is its symbol, `block` the
  * workgroup size it was compiled for, `flags` a combination of the
@@ -89,7 +90,10 @@ int64_t mdx_workspace_bytes(int64_t handle, int64_t n_traj);
  * `grid` 0 sizes a one-lane-per-trajectory launch from N, or a persistent
  * kernel's resident grid. `weights` may be NULL only for a kernel registered
  * with MDX_KERNEL_UNIT_WEIGHTS. `workspace` must hold mdx_workspace_bytes; it
- * is reset on the stream before the solve. */
+ * is reset on the stream before the solve. A kernel registered with
+ * MDX_KERNEL_HOOK zeroes and accumulates hook_out (N, hook_size), laid out like
+ * params; one compiled without the history writes only hist's first save row
+ * (the final state), so hist may hold that one row. */
 int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_traj,
                          int32_t n_vars, int32_t n_params, int32_t n_save, int32_t layout,
                          const double* y0, const double* times, const double* params, double dt0,

@@ -482,6 +482,100 @@ class _Vector:
         return iter(self._items)
 
 
+class _HookAcc:
+    """The hook's accumulator row as the hook sees it while traced: reads of a
+    slot not yet written this call are inputs, writes are recorded in order."""
+
+    def __init__(self, g: Graph, size: int):
+        self._g = g
+        self._size = size
+        self.writes: dict[int, Node] = {}
+
+    def _slot(self, k):
+        if isinstance(k, Sym):
+            raise TraceError("the hook's accumulator must be indexed with constants or save_idx")
+        k = int(k)
+        if not 0 <= k < self._size:
+            raise IndexError(f"hook accumulator index {k} outside hook_size {self._size}")
+        return k
+
+    def __getitem__(self, k):
+        k = self._slot(k)
+        node = self.writes.get(k)
+        return Sym(self._g, self._g.inp("acc", k) if node is None else node)
+
+    def __setitem__(self, k, v):
+        k = self._slot(k)
+        if isinstance(v, Sym):
+            self.writes[k] = v.n
+        elif isinstance(v, (bool, int, float, np.floating, np.integer)):
+            self.writes[k] = self._g.const(float(v))
+        else:
+            raise TraceError(f"the hook stored a value of type {type(v).__name__}")
+
+    def __len__(self):
+        return self._size
+
+
+@dataclass
+class TracedHook:
+    """A save hook traced once per save index: ``cases[s]`` is the list of
+    ``(slot, node)`` stores the call at save ``s`` makes."""
+
+    graph: Graph
+    cases: list
+    n_vars: int
+    n_params: int
+    hook_size: int
+
+
+def trace_hook(hook, n_vars: int, n_params: int, n_save: int, hook_size: int) -> TracedHook:
+    """Record ``hook(save_idx, y, t, p_row, acc)`` (modax/rodas5P.py:157-164,
+    976-1002) for every save index: the index is a constant in each trace, so
+    ``acc[save_idx]`` and host arrays indexed by it resolve at trace time."""
+    g = Graph()
+    ys = _Vector([Sym(g, g.y(i)) for i in range(n_vars)])
+    ps = _Vector([Sym(g, g.p(j)) for j in range(n_params)])
+    ts = Sym(g, g.t())
+    fn = _with_traceable_globals(hook)
+    saved = {name: getattr(math, name) for name in _PATCHED_MATH}
+    cases = []
+    try:
+        for name in _PATCHED_MATH:
+            setattr(math, name, getattr(_MATH_SHIM, name))
+        for sv in range(n_save):
+            acc = _HookAcc(g, hook_size)
+            fn(sv, ys, ts, ps, acc)
+            cases.append(sorted(acc.writes.items()))
+    finally:
+        for name, value in saved.items():
+            setattr(math, name, value)
+    return TracedHook(g, cases, n_vars, n_params, hook_size)
+
+
+def generate_hook(th: TracedHook) -> str:
+    """``save_hook(save_idx, y, t, p, acc)`` as a switch over the save index,
+    save indices whose traces are identical sharing one case."""
+    groups: dict = {}
+    for sv, writes in enumerate(th.cases):
+        groups.setdefault(tuple((k, n.id) for k, n in writes), []).append(sv)
+    lines = ["template <class YA, class PA, class AC>",
+             "__device__ __forceinline__ void save_hook(int save_idx, const YA& y, const double t,",
+             "                                          const PA& p, const AC& acc) {",
+             "  (void)y;", "  (void)t;", "  (void)p;", "  (void)acc;", "  switch (save_idx) {"]
+    for key, svs in groups.items():
+        if not key:
+            continue
+        nodes = [th.graph.nodes[nid] for _, nid in key]
+        body, _ = Emitter(th.graph).body(nodes, indent="      ")
+        lines.append("    " + " ".join(f"case {sv}:" for sv in svs) + " {")
+        lines += body
+        lines += [f"      acc.put({k}, v{nid});" for k, nid in key]
+        lines += ["      break;", "    }"]
+    lines += ["    default:", "      break;", "  }", "}"]
+    return "\n".join(lines) + "\n"
+
+
 def _with_traceable_globals(fn):
     """A copy of ``fn`` whose globals see ``min``/``max``/``where`` and the math shim."""
     if not isinstance(fn, types.FunctionType):

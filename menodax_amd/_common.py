@@ -112,6 +112,7 @@ class Outputs:
     accepted: object
     rejected: object
     loop: object
+    hook_out: object = None  # (hook_size, N) storage when a save hook runs
 
     def ensemble_view(self):
         """``(N, n_save, n_vars)`` view of the structure-of-arrays history."""

@@ -66,6 +66,13 @@ class KernelOptions:
     placement: str | None = None
     #: forward sensitivities carried with the state (a ``SensitivitySpec``), or None
     sens: object = None
+    #: a save hook ``hook(save_idx, y, t, p_row, acc)`` (modax/rodas5P.py:157-164),
+    #: the width of its accumulator row, whether the history is kept, and the
+    #: number of save times the hook was traced for
+    save_hook: object = None
+    hook_size: int = 0
+    save_history: bool = True
+    hook_n_save: int = 0
 
 
 def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None) -> dict:
@@ -156,8 +163,10 @@ def _params_in_regs(options, n_params):
     return n_params <= PARAMS_IN_REGS_MAX
 
 
-def _register(code, block, unit_weights, plain_entry, refill_entry):
+def _register(code, block, unit_weights, plain_entry, refill_entry, hook=False):
     flags = _runtime.KERNEL_UNIT_WEIGHTS if unit_weights else 0
+    if hook:
+        flags |= _runtime.KERNEL_HOOK
     plain = _runtime.kernel_create(code, plain_entry, block, flags)
     refill = _runtime.kernel_create(code, refill_entry, block, flags | _runtime.KERNEL_PERSISTENT)
     return plain, refill
@@ -232,6 +241,19 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
         "MDX_MAX_VGPRS": _max_vgprs(options.max_registers),
     }
     sens_src = ""
+    if options.save_hook is not None or not options.save_history:
+        if adjoint or spec is not None:
+            raise ValueError("a solve with a save hook is not differentiated")
+        if options.save_hook is None:
+            raise ValueError("save_history=False needs a save_hook to consume the saves")
+        if options.hook_size <= 0:
+            raise ValueError("a save_hook needs a positive hook_size")
+        th = _codegen.trace_hook(options.save_hook, n_vars, n_params, options.hook_n_save,
+                                 options.hook_size)
+        defs["MDX_HOOK"] = 1
+        defs["MDX_HOOK_SIZE"] = int(options.hook_size)
+        defs["MDX_SAVE_HISTORY"] = 1 if options.save_history else 0
+        sens_src = _codegen.generate_hook(th)
     if adjoint:
         defs["MDX_ADJOINT"] = 1
         sens_src = _codegen.generate_adjoint(traced)
@@ -253,7 +275,8 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
 def rodas5p_kernel(ode_fn, n_vars: int, n_params: int, options: KernelOptions) -> Built:
     source, d = rodas5p_source(ode_fn, n_vars, n_params, options)
     code = _runtime.code_object(source, f"rodas5p_{getattr(ode_fn, '__name__', 'rhs')}.hip")
-    plain, refill = _register(code, d["block"], options.unit_weights, "mdx_rodas5p", "mdx_rodas5p_refill")
+    plain, refill = _register(code, d["block"], options.unit_weights, "mdx_rodas5p",
+                              "mdx_rodas5p_refill", hook=options.save_hook is not None)
     return Built(
         method="rodas5p", plain=plain, refill=refill, block=d["block"], n_vars=n_vars,
         n_params=n_params, unit_weights=options.unit_weights, source=source,

@@ -24,6 +24,7 @@ ARCH = "gfx950"
 KERNEL_PERSISTENT = 1
 KERNEL_UNIT_WEIGHTS = 2
 KERNEL_ADJOINT = 4
+KERNEL_HOOK = 8
 
 # Every kernel is compiled with contraction off (the fused multiply-adds are
 # explicit in the source) and without packed FP32 instructions.

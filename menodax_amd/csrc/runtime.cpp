@@ -47,10 +47,11 @@ struct SolveArgs {
   int64_t p_ts, p_vs;
   int64_t w_ts, w_vs;
   int64_t h_ts, h_ss, h_vs;
+  int64_t ho_ts, ho_vs;
   int32_t max_steps;
   int32_t n_save;
 };
-static_assert(sizeof(SolveArgs) == 10 * 8 + 3 * 8 + 8 + 9 * 8 + 8, "SolveArgs layout");
+static_assert(sizeof(SolveArgs) == 10 * 8 + 3 * 8 + 8 + 11 * 8 + 8, "SolveArgs layout");
 
 // Mirrors mdx::r5::AdjointArgs in kernels/rodas5p_adjoint.hpp.
 struct AdjointArgs {
@@ -92,6 +93,7 @@ struct Kernel {
   bool persistent = false;
   bool unit_weights = false;
   bool adjoint = false;
+  bool hook = false;
   std::map<int, Loaded> per_device;
 };
 
@@ -200,6 +202,7 @@ int32_t mdx_kernel_create(const void* code, size_t code_size, const char* entry,
   k->persistent = (flags & MDX_KERNEL_PERSISTENT) != 0;
   k->unit_weights = (flags & MDX_KERNEL_UNIT_WEIGHTS) != 0;
   k->adjoint = (flags & MDX_KERNEL_ADJOINT) != 0;
+  k->hook = (flags & MDX_KERNEL_HOOK) != 0;
   if (k->adjoint && k->persistent)
     return fail(1, "mdx_kernel_create: an adjoint kernel is not a persistent kernel");
   std::lock_guard<std::mutex> lock(g_mutex);
@@ -280,6 +283,8 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
   if (!weights && !k->unit_weights)
     return fail(1, "mdx_solve_launch: this kernel reads error weights; weights is NULL");
   if (hook_size > 0 && !hook_out) return fail(1, "mdx_solve_launch: hook_size > 0 needs hook_out");
+  if (k->hook && (hook_size <= 0 || !hook_out))
+    return fail(1, "mdx_solve_launch: this kernel runs a save hook; hook_out and hook_size are required");
   if (max_steps < 0) return fail(1, "mdx_solve_launch: max_steps must be non-negative");
   if (k->adjoint) return fail(1, "mdx_solve_launch: an adjoint kernel is launched by mdx_adjoint_launch");
   if (k->persistent && (!workspace || workspace_bytes < 8))
@@ -309,11 +314,13 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
     a.p_ts = n_params, a.p_vs = 1;
     a.w_ts = n_vars, a.w_vs = 1;
     a.h_ts = (int64_t)n_save * n_vars, a.h_ss = n_vars, a.h_vs = 1;
+    a.ho_ts = hook_size, a.ho_vs = 1;
   } else {  // structure of arrays: trajectory index fastest
     a.y0_ts = 1, a.y0_vs = n_traj;
     a.p_ts = 1, a.p_vs = n_traj;
     a.w_ts = 1, a.w_vs = n_traj;
     a.h_ts = 1, a.h_ss = (int64_t)n_vars * n_traj, a.h_vs = n_traj;
+    a.ho_ts = 1, a.ho_vs = n_traj;
   }
 
   Loaded* l = nullptr;

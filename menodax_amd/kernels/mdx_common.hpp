@@ -29,7 +29,7 @@ struct SolveArgs {
   int* accepted;          // (traj,)
   int* rejected;
   int* loop;
-  double* hook_out;       // reserved
+  double* hook_out;       // (traj, hook_size): save-hook accumulators
   u64* queue;             // persistent refill: next unclaimed trajectory
   double dt0, rtol, atol;
   i64 n_traj;
@@ -37,8 +37,17 @@ struct SolveArgs {
   i64 p_ts, p_vs;
   i64 w_ts, w_vs;
   i64 h_ts, h_ss, h_vs;
+  i64 ho_ts, ho_vs;       // hook_out strides
   int max_steps;
   int n_save;
+};
+
+// A trajectory's save-hook accumulator row in global memory.
+struct HookRow {
+  double* base;
+  i64 stride;
+  __device__ __forceinline__ double operator[](int k) const { return base[k * stride]; }
+  __device__ __forceinline__ void put(int k, double v) const { base[k * stride] = v; }
 };
 
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }

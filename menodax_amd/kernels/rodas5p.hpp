@@ -27,6 +27,9 @@
 //                        MDX_N * (1 + MDX_NSENS) long, the matrix stays n x n
 //   MDX_SENS_PCOLS       per direction, the parameter it seeds (-1: a
 //                        direction of the initial state)
+//   MDX_HOOK, MDX_HOOK_SIZE  call sys::save_hook at every save with the saved
+//                        state, accumulating into the trajectory's hook_out row
+//   MDX_SAVE_HISTORY     0: write no history, only the final state in row 0
 //
 // The method, stage by stage, is the reference kernel's (modax/rodas5P.py:
 // tableau 44-105, controller 108-113, 340-342, loop 761-989, tail 991-1005);
@@ -92,6 +95,15 @@ __device__ constexpr double H3[8] = {11.419903575922262, 2.8879645146136994, 72.
 #endif
 #ifndef MDX_ADJOINT
 #define MDX_ADJOINT 0
+#endif
+#ifndef MDX_HOOK
+#define MDX_HOOK 0
+#endif
+#ifndef MDX_SAVE_HISTORY
+#define MDX_SAVE_HISTORY 1
+#endif
+#if MDX_HOOK && MDX_ADJOINT
+#error "a save hook is not differentiated"
 #endif
 // The rolled kernel copies the parameter row into per-lane scratch: each read
 // is then one load at a constant offset, with no address kept in registers.
@@ -580,9 +592,20 @@ __device__ __forceinline__ void init(State& s, const SolveArgs& a, i64 i, const 
   const double tf = times[a.n_save - 1];
 #pragma unroll
   for (int j = 0; j < NA; ++j) YJ(s, j) = a.y0[i * a.y0_ts + (i64)j * a.y0_vs];
-#if !MDX_ADJOINT
+#if !MDX_ADJOINT && MDX_SAVE_HISTORY
 #pragma unroll
   for (int j = 0; j < NA; ++j) a.hist[i * a.h_ts + (i64)j * a.h_vs] = YJ(s, j);
+#endif
+#if MDX_HOOK
+  {
+    const HookRow acc{a.hook_out + i * a.ho_ts, a.ho_vs};
+    for (int k = 0; k < MDX_HOOK_SIZE; ++k) acc.put(k, 0.0);
+#if MDX_LDS_Y
+    sys::save_hook(0, LdsVec{&lds_y[lane_]}, times[0], p, acc);
+#else
+    sys::save_hook(0, s.y, times[0], p, acc);
+#endif
+  }
 #endif
   s.t = times[0];
   s.dt = a.dt0 > 0.0 ? a.dt0 : (tf - times[0]) * 1e-6;
@@ -814,6 +837,9 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
       const double theta = (ts - t) / dt_use;
       const double theta1 = 1.0 - theta;
       double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
+#if MDX_HOOK
+      double ysave[NA];
+#endif
 #pragma unroll
       for (int j = 0; j < NA; ++j, row += a.h_vs) {
         const double k0 = (double)KQ(0, j);
@@ -831,8 +857,17 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
         double v = fmaT(theta, h3, h2);
         v = fmaT(theta, v, h1);
         v = fmaT(theta1, v, y_new);
-        *row = fmaT(theta, v, theta1 * YJ(s, j));
+        const double val = fmaT(theta, v, theta1 * YJ(s, j));
+#if MDX_SAVE_HISTORY
+        *row = val;
+#endif
+#if MDX_HOOK
+        ysave[j] = val;
+#endif
       }
+#if MDX_HOOK
+      sys::save_hook(s.save_idx, ysave, ts, p, HookRow{a.hook_out + i * a.ho_ts, a.ho_vs});
+#endif
 #endif
       s.save_idx++;
     }
@@ -847,14 +882,30 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
 }
 
 // Saves past the trajectory's end hold its final state; counters out.
-__device__ __forceinline__ void finish(State& s, const SolveArgs& a, i64 i) {
+__device__ __forceinline__ void finish(State& s, const PA& p, const SolveArgs& a, i64 i) {
   MDX_LANE();
+  (void)p;
   while (s.save_idx < a.n_save) {
+#if MDX_SAVE_HISTORY
     double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
 #pragma unroll
     for (int j = 0; j < NA; ++j) row[(i64)j * a.h_vs] = YJ(s, j);
+#endif
+#if MDX_HOOK
+    const HookRow acc{a.hook_out + i * a.ho_ts, a.ho_vs};
+#if MDX_LDS_Y
+    sys::save_hook(s.save_idx, LdsVec{&lds_y[lane_]}, a.times[s.save_idx], p, acc);
+#else
+    sys::save_hook(s.save_idx, s.y, a.times[s.save_idx], p, acc);
+#endif
+#endif
     s.save_idx++;
   }
+#if !MDX_SAVE_HISTORY
+  // without the history the one row holds the final state (rodas5P.py:999-1002)
+#pragma unroll
+  for (int j = 0; j < NA; ++j) a.hist[i * a.h_ts + (i64)j * a.h_vs] = YJ(s, j);
+#endif
   a.accepted[i] = s.accepted;
   a.rejected[i] = s.rejected;
   a.loop[i] = s.n_steps;
@@ -880,7 +931,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p(
   State s;
   init(s, a, i, p);
   while (active(s, a)) step(s, p, a, i);
-  finish(s, a, i);
+  finish(s, p, a, i);
 }
 
 // Persistent refill: a resident grid whose lanes claim trajectories from a
@@ -912,7 +963,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p_
     if (i >= 0) {
       if (active(s, a)) step(s, p, a, i);
       if (!active(s, a)) {
-        finish(s, a, i);
+        finish(s, p, a, i);
         i = -1;
       }
     }

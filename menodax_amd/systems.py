@@ -343,6 +343,27 @@ def damped_oscillator(y, t, p):
     return (y[1], -p[0] * y[0] - p[1] * y[1])
 
 
+def trapezoid_hook(t_span):
+    """A save hook integrating the state over the save grid by the trapezoid
+    rule, and counting the saves (tests/test_save_hook.py:48-52)."""
+    t = np.asarray(t_span, dtype=np.float64)
+    w = np.zeros_like(t)
+    w[1:] += 0.5 * np.diff(t)
+    w[:-1] += 0.5 * np.diff(t)
+
+    def integrate_state(save_idx, y, t_, p_row, acc):
+        acc[0] += w[save_idx] * y[0]
+        acc[1] += w[save_idx] * y[1]
+        acc[2] += 1.0
+
+    return integrate_state
+
+
+def energy_hook(save_idx, y, t, p_row, acc):
+    """A save hook storing the oscillator's energy per save (tests/test_save_hook.py:80-81)."""
+    acc[save_idx] = 0.5 * (y[1] * y[1] + p_row[0] * y[0] * y[0])
+
+
 def kaps(n_pairs, epsilon_min):
     """kaps.py:26-48; exact solution y1 = exp(-2t), y2 = exp(-t) for p[0] = 1."""
     epsilon = [10.0 ** (np.log10(epsilon_min) * i / (n_pairs - 1)) for i in range(n_pairs)]

@@ -224,3 +224,24 @@ def test_every_placement_compiles(placement):
     src, d = _kernels.rodas5p_source(fn, 12, S.arrow_n_params(12), opts)
     assert d["placement"]["placement"] == placement
     assert _runtime.rtc_compile(src, f"arrow_{placement}.hip")[:4] == b"\x7fELF"
+
+
+def test_save_history_false_needs_a_hook():
+    from menodax_amd import rodas5P
+
+    with pytest.raises(ValueError, match="save_history=False"):
+        rodas5P.solve(S.damped_oscillator, [1.0, 0.0], [0.0, 1.0], [4.0, 0.2], save_history=False)
+    with pytest.raises(ValueError, match="positive hook_size"):
+        rodas5P.solve(S.damped_oscillator, [1.0, 0.0], [0.0, 1.0], [4.0, 0.2],
+                      save_hook=S.energy_hook, hook_size=0)
+
+
+def test_hook_trace_resolves_the_save_index_and_shares_identical_cases():
+    th = cg.trace_hook(S.trapezoid_hook(np.linspace(0, 1, 5)), 2, 2, 5, 3)
+    src = cg.generate_hook(th)
+    # the two ends share one trapezoid weight and the interior saves another: two cases
+    assert src.count("acc.put(2,") == 2 and "case 1: case 2: case 3:" in src
+    th = cg.trace_hook(S.energy_hook, 2, 2, 4, 4)
+    assert [k for k, _ in th.cases[3]] == [3]
+    with pytest.raises(IndexError, match="hook_size"):
+        cg.trace_hook(S.energy_hook, 2, 2, 4, 3)
