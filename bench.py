@@ -26,7 +26,12 @@ The JSON line also carries
   - ``uneven_stiff``: the 100,000-trajectory uneven stiff ensemble (50-variable
     block-arrow system, step counts spread over two decades): the FP32-LU
     persistent-refill solve against the plain FP64 one-lane solve, with the
-    endpoint error against FP64 oracle solves.
+    endpoint error against FP64 oracle solves;
+  - ``adjoint``: gradients of a squared-error loss over 20 saves with respect to
+    mu and y0 for 8,192 stiff Van der Pol trajectories (2 variables, and the
+    20-variable chain with its sparse LU), FP64, by the discrete-adjoint kernel
+    (checkpoint every 64 accepted steps) and by the forward-sensitivity joint
+    solve, with their agreement and a central-difference check on a sample.
 
 ``python bench.py --gpus N`` with N > 1 exits: this build runs on one GPU.
 """
@@ -182,6 +187,77 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64):
     }
 
 
+def adjoint_block(torch, log_fn, n=8192, ckpt_every=64):
+    """C4: d(loss)/d(mu, y0) for stiff Van der Pol ensembles."""
+    from menodax_amd import rodas5P, systems as S
+
+    out = {"workload": f"{n} stiff Van der Pol trajectories, mu ~ LogUniform[10, 100], "
+                       "y0 ~ U[-2, 2], t in [0, 3], 20 saves, loss = sum of squared errors vs a "
+                       "target solved at 1.1 mu plus noise (seed 0), fp64, rtol 1e-8 atol 1e-10",
+           "ckpt_every": ckpt_every}
+    for n_osc in (1, 10):
+        fn, _ = S.vdp_lattice(n_osc, mu=1.0, d=0.1, wrap=False)
+        nv = 2 * n_osc
+        sparsity = None if n_osc == 1 else S.vdp_sparsity(n_osc, wrap=False)
+        rng = np.random.default_rng(0)
+        mu = 10.0 ** rng.uniform(1.0, 2.0, size=(n, 1))
+        y0 = rng.uniform(-2.0, 2.0, size=(n, nv))
+        t = np.linspace(0.0, 3.0, 20)
+        kw = dict(rtol=1e-8, atol=1e-10, lu_precision="fp64", sparsity=sparsity)
+        target = rodas5P.solve(fn, y0, t, mu * 1.1, **kw) + 0.01 * rng.normal(size=(n, 20, nv))
+        dev = torch.device("cuda", 0)
+        y0d, mud = torch.tensor(y0, device=dev), torch.tensor(mu, device=dev)
+        tgt = torch.tensor(target, device=dev)
+
+        def run_fwd():
+            return rodas5P.solve(fn, y0d, t, mud, **kw)
+
+        hist = run_fwd()
+        g = 2.0 * (hist - tgt)
+
+        def run_adj():
+            return rodas5P.adjoint_gradients(fn, y0d, t, mud, g, ckpt_every=ckpt_every, **kw)
+
+        def run_sens():
+            return rodas5P.solve_sensitivities(fn, y0d, t, mud, **kw)
+
+        def clock(f, reps=3):
+            f()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                r = f()
+            torch.cuda.synchronize()
+            return r, (time.perf_counter() - t0) / reps
+
+        _, t_fwd = clock(run_fwd)
+        (gy, gp), t_adj = clock(run_adj)
+        (hs, sens), t_sens = clock(run_sens)
+        gdir = torch.einsum("nsi,nsik->nk", 2.0 * (hs - tgt), sens)
+        ref = torch.cat([gdir[:, :nv], gdir[:, nv:]], 1)
+        got = torch.cat([gy, gp], 1)
+        rel_sens = float(((got - ref).abs().max(1).values / ref.abs().max(1).values).max())
+        # central differences of the loss in mu on a small sample
+        k = 4
+        h = 1e-6 * mu[:k]
+        lp = ((rodas5P.solve(fn, y0[:k], t, mu[:k] + h, **kw) - target[:k]) ** 2).sum((1, 2))
+        lm = ((rodas5P.solve(fn, y0[:k], t, mu[:k] - h, **kw) - target[:k]) ** 2).sum((1, 2))
+        fd = (lp - lm) / (2 * h[:, 0])
+        rel_fd = float(np.max(np.abs(gp[:k, 0].cpu().numpy() - fd) / np.abs(fd)))
+        key = f"vdp{nv}"
+        out[key] = {
+            "n_vars": nv, "lu": "dense" if sparsity is None else "sparse (AMD)",
+            "forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3,
+            "forward_sensitivity_ms": t_sens * 1e3,
+            "gradients_per_s": n / (t_fwd + t_adj),
+            "adjoint_vs_sensitivity_max_rel": rel_sens,
+            "adjoint_vs_central_difference_max_rel": rel_fd,
+        }
+        log_fn(f"adjoint {key}: fwd {t_fwd*1e3:.1f} ms adj {t_adj*1e3:.1f} ms sens {t_sens*1e3:.1f} ms "
+               f"rel {rel_sens:.2e} fd {rel_fd:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +271,8 @@ def main():
     ap.add_argument("--skip-reference-config", action="store_true")
     ap.add_argument("--skip-uneven-stiff", action="store_true",
                     help="leave out the 100k uneven stiff ensemble comparison")
+    ap.add_argument("--skip-adjoint", action="store_true",
+                    help="leave out the adjoint-gradient block")
     ap.add_argument("--only-uneven-stiff", action="store_true",
                     help="run and print only the uneven stiff ensemble comparison (for profiling)")
     ap.add_argument("--lu", choices=("both", "fp32", "fp64"), default="both",
@@ -413,6 +491,8 @@ def main():
 
     if not args.skip_uneven_stiff:
         line["uneven_stiff"] = uneven_stiff(torch, dev, O, log)
+    if not args.skip_adjoint:
+        line["adjoint"] = adjoint_block(torch, log)
 
     print(json.dumps(line), flush=True)
 

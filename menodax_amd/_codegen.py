@@ -482,6 +482,61 @@ class _Vector:
         return iter(self._items)
 
 
+class _OutRecorder:
+    """The ``out`` array an array-form right-hand side writes into while traced."""
+
+    def __init__(self, g: Graph, n: int):
+        self._g = g
+        self._n = n
+        self.values: dict[int, Node] = {}
+
+    def __setitem__(self, k, v):
+        if isinstance(k, Sym):
+            raise TraceError("out must be indexed with constants")
+        k = int(k)
+        if not 0 <= k < self._n:
+            raise IndexError(f"out index {k} outside the {self._n}-variable state")
+        if isinstance(v, Sym):
+            self.values[k] = v.n
+        elif isinstance(v, (bool, int, float, np.floating, np.integer)):
+            self.values[k] = self._g.const(float(v))
+        else:
+            raise TraceError(f"out[{k}] was given a value of type {type(v).__name__}")
+
+    def __getitem__(self, k):
+        k = int(k)
+        if k not in self.values:
+            raise TraceError(f"out[{k}] read before it was written")
+        return Sym(self._g, self.values[k])
+
+    def __len__(self):
+        return self._n
+
+
+def trace_array_rhs(array_rhs, n_vars: int, n_params: int) -> Traced:
+    """Record an array-form right-hand side ``array_rhs(y, t, p, out)``
+    (modax/rodas5P.py:382-392): every component of ``out`` must be written."""
+    g = Graph()
+    ys = [Sym(g, g.y(i)) for i in range(n_vars)]
+    ps = [Sym(g, g.p(j)) for j in range(n_params)]
+    ts = Sym(g, g.t())
+    out = _OutRecorder(g, n_vars)
+    fn = _with_traceable_globals(array_rhs)
+    saved = {name: getattr(math, name) for name in _PATCHED_MATH}
+    try:
+        for name in _PATCHED_MATH:
+            setattr(math, name, getattr(_MATH_SHIM, name))
+        fn(_Vector(ys), ts, _Vector(ps), out)
+    finally:
+        for name, value in saved.items():
+            setattr(math, name, value)
+    missing = [j for j in range(n_vars) if j not in out.values]
+    if missing:
+        raise ValueError(f"array_rhs left out components {missing} of the {n_vars}-variable state")
+    nodes = [out.values[j] for j in range(n_vars)]
+    return Traced(g, nodes, n_vars, n_params, any(n.op == "t" for n in g.nodes))
+
+
 class _HookAcc:
     """The hook's accumulator row as the hook sees it while traced: reads of a
     slot not yet written this call are inputs, writes are recorded in order."""

@@ -11,7 +11,7 @@ trajectory and the persistent refill kernel -- come from the same code object.
 from __future__ import annotations
 
 import functools
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 from . import _codegen, _runtime
 from ._sparsity import CompressedJacobian, check_superset, colour_sparsity, dense_jacobian
@@ -66,6 +66,9 @@ class KernelOptions:
     placement: str | None = None
     #: forward sensitivities carried with the state (a ``SensitivitySpec``), or None
     sens: object = None
+    #: an array-form right-hand side ``f(y, t, p, out)`` for the stage
+    #: evaluations; the Jacobian still comes from ``ode_fn`` (rodas5P.py:382-392)
+    array_rhs: object = None
     #: a save hook ``hook(save_idx, y, t, p_row, acc)`` (modax/rodas5P.py:157-164),
     #: the width of its accumulator row, whether the history is kept, and the
     #: number of save times the hook was traced for
@@ -205,6 +208,14 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
         traced, colour=structure.colour, n_colours=structure.n_colours,
         store=structure.stores(),
     )
+    if options.array_rhs is not None and options.sens is None:
+        # the joint system's rows come from ode_fn's sweeps, as in the reference
+        arr = _codegen.trace_array_rhs(options.array_rhs, n_vars, n_params)
+        rhs_src, rhs_fc = _codegen.generate_rhs(arr)
+        jac_src, _ = _codegen.generate_neg_jac(
+            traced, colour=structure.colour, n_colours=structure.n_colours,
+            store=structure.stores())
+        gen = replace(gen, source=rhs_src + "\n\n" + jac_src + "\n", rhs_flops=rhs_fc)
     if options.tf_index >= n_params:
         raise ValueError(f"tf_index {options.tf_index} is not a column of the {n_params} parameters")
     e1, e2, e3 = pid_exponents(RODAS_EXPONENT, options.pcoeff, options.icoeff, options.dcoeff)

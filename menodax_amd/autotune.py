@@ -1,0 +1,102 @@
+"""Autotuning the Rodas5P launch: placement, workgroup size, launch shape.
+
+The placements (registers / rolled scratch / rolled LDS), the trajectories per
+workgroup and the two launch shapes (one lane per trajectory, persistent
+refill) move data and work, never arithmetic, so every candidate computes the
+same bits. Which is fastest depends on the system size, the register and LDS
+footprint and the spread of step counts, and is measured rather than guessed:
+``tune`` times each candidate on a sample of the ensemble, checks that they all
+agree bit for bit, and returns the winner's options, cached per (callback,
+sizes, options) for the process.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, replace
+
+from . import _common, _kernels
+from ._kernels import PLACEMENTS, KernelOptions, rodas5p_placement
+from ._sparsity import normalize_sparsity
+from .rodas5P import launch
+
+_CACHE: dict = {}
+
+
+@dataclass(frozen=True)
+class Choice:
+    placement: str
+    trajectories_per_block: int
+    refill: bool
+    seconds: float
+
+
+def candidates(n_vars: int, lu_precision: str, blocks=(64, 128, 256)):
+    """(placement, block) pairs worth timing: the unrolled register kernel only
+    where it can plausibly fit, the LDS placement only at block sizes where
+    something fits in LDS."""
+    out = []
+    for placement in PLACEMENTS:
+        if placement == "registers" and n_vars > 32:
+            continue
+        for b in blocks:
+            if placement == "lds" and rodas5p_placement(n_vars, lu_precision, b, "lds")["lds_bytes"] == 0:
+                continue
+            out.append((placement, b))
+    return out
+
+
+def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-10,
+         first_step=None, max_steps=100000, lu_precision="fp32", sparsity=None, ordering="amd",
+         pcoeff=0.0, icoeff=1.0, dcoeff=0.0, tf_index=None, blocks=(64, 128, 256)):
+    """Time every candidate on ``sample`` trajectories (all of them by default)
+    and return ``(best Choice, [Choice for every candidate])``. Raises if two
+    candidates disagree in any bit."""
+    import numpy as np
+    import torch
+
+    dev = _common.device()
+    y0_t, p_t, n, n_vars = _common.normalize_y0_params(y0, params, dev)
+    if sample is not None and sample < n:
+        y0_t, p_t, n = y0_t[:sample], p_t[:sample], int(sample)
+    n_params = p_t.shape[1]
+    if sparsity is not None:
+        sparsity = normalize_sparsity(sparsity, n_vars)
+    base = KernelOptions(pcoeff=float(pcoeff), icoeff=float(icoeff), dcoeff=float(dcoeff),
+                         lu_precision=lu_precision, sparsity=sparsity, ordering=ordering,
+                         tf_index=-1 if tf_index is None else int(tf_index))
+    key = (ode_fn, n_vars, n_params, base, n, tuple(blocks))
+    if key in _CACHE:
+        return _CACHE[key]
+    times = _common.as_f64_device(t_span, dev).contiguous()
+    kw = dict(dt0=_common.initial_step(first_step), rtol=rtol, atol=atol, max_steps=max_steps)
+    results, reference = [], None
+    for placement, block in candidates(n_vars, lu_precision, blocks):
+        built = _kernels.rodas5p_kernel(
+            ode_fn, n_vars, n_params,
+            replace(base, placement=placement, trajectories_per_block=block))
+        for refill in (False, True):
+            out = launch(built, y0_t, times, p_t, n, refill=refill, **kw)
+            torch.cuda.synchronize()
+            hist = out.hist.cpu().numpy()
+            if reference is None:
+                reference = hist
+            elif not np.array_equal(hist, reference, equal_nan=True):
+                raise RuntimeError(f"placement {placement}, block {block}, refill {refill} "
+                                   "changed the result; placements must be bit-identical")
+            ws = out._keep[-1]
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                launch(built, y0_t, times, p_t, n, refill=refill, outputs=out, workspace=ws, **kw)
+            torch.cuda.synchronize()
+            results.append(Choice(placement, block, refill, (time.perf_counter() - t0) / reps))
+    best = min(results, key=lambda c: c.seconds)
+    _CACHE[key] = (best, results)
+    return best, results
+
+
+def tuned_options(ode_fn, y0, t_span, params, **kw) -> dict:
+    """``solve`` keyword arguments for the fastest candidate (see ``tune``)."""
+    best, _ = tune(ode_fn, y0, t_span, params, **kw)
+    return {"placement": best.placement, "trajectories_per_block": best.trajectories_per_block,
+            "refill": best.refill}

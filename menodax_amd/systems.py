@@ -85,6 +85,14 @@ def robertson(y, t, p):
     )
 
 
+def robertson_array(y, t, p, out):
+    """Robertson as an array-form right-hand side (``array_rhs``): the same
+    expressions in the same order as ``robertson``, written into ``out``."""
+    out[0] = -p[0] * y[0] + p[1] * y[1] * y[2]
+    out[1] = p[0] * y[0] - p[1] * y[1] * y[2] - p[2] * y[1] ** 2
+    out[2] = p[2] * y[1] ** 2
+
+
 def robertson_initial_conditions(size, seed=42, *, divergence=1.0):
     divergence = _check_divergence(divergence)
     if divergence == 0.0:

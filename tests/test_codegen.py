@@ -245,3 +245,25 @@ def test_hook_trace_resolves_the_save_index_and_shares_identical_cases():
     assert [k for k, _ in th.cases[3]] == [3]
     with pytest.raises(IndexError, match="hook_size"):
         cg.trace_hook(S.energy_hook, 2, 2, 4, 3)
+
+
+def test_array_rhs_traces_to_the_same_expressions():
+    tr = cg.trace(S.robertson, 3, 3)
+    arr = cg.trace_array_rhs(S.robertson_array, 3, 3)
+    rng = np.random.default_rng(7)
+    for _ in range(10):
+        y, p = rng.uniform(0, 1, 3), rng.uniform(0.5, 2, 3)
+        assert np.array_equal(cg.rhs_from_graph(arr, y, 0.0, p), cg.rhs_from_graph(tr, y, 0.0, p))
+
+    def partial(y, t, p, out):
+        out[0] = y[0]
+
+    with pytest.raises(ValueError, match="left out components"):
+        cg.trace_array_rhs(partial, 2, 0)
+
+
+def test_array_rhs_kernel_compiles_with_the_jacobian_of_ode_fn():
+    opts = _kernels.KernelOptions(array_rhs=S.robertson_array)
+    src, _ = _kernels.rodas5p_source(S.robertson, 3, 3, opts)
+    assert "void rhs(" in src and "void neg_jac(" in src
+    assert _runtime.rtc_compile(src, "arr.hip")[:4] == b"\x7fELF"

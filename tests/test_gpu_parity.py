@@ -268,6 +268,17 @@ def test_every_placement_is_bit_exact(placement, lu):
         _same(hist, st, ref, rst)
 
 
+def test_array_rhs_stage_evaluations_are_bit_identical():
+    """The array-form callback computing the same expressions drives the eight
+    stage evaluations; the Jacobian still comes from ode_fn."""
+    y0, p = S.robertson_lognormal_ensemble(500, seed=6)
+    t = S.robertson_benchmark_times()
+    kw = dict(rtol=1e-6, atol=1e-8, return_stats=True)
+    ref, rst = rodas5P.solve(S.robertson, y0, t, p, **kw)
+    hist, st = rodas5P.solve(S.robertson, y0, t, p, array_rhs=S.robertson_array, **kw)
+    _same(hist, st, ref, {k: np.asarray(v) for k, v in rst.items()})
+
+
 # --- Tsit5 --------------------------------------------------------------------
 @pytest.mark.parametrize("refill", [False, True])
 def test_lorenz_rho_ensemble_bit_exact(refill):
