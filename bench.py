@@ -187,6 +187,68 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64):
     }
 
 
+def lorenz_block(torch, dev, n=1024):
+    """C1: Tsit5 on the Lorenz ensemble -- rho ~ U[0, 28] (seed 0), y0 = (1, 0, 0),
+    t in [0, 1] with 100 saves, rtol 1e-6 atol 1e-8."""
+    from menodax_amd import _kernels, rodas5P, systems as S
+
+    y0, p = S.lorenz_rho_ensemble(n)
+    t = torch.tensor(np.linspace(0.0, 1.0, 100), device=dev)
+    y0_t = torch.tensor(np.ascontiguousarray(y0.T), device=dev).t()
+    p_t = torch.tensor(np.ascontiguousarray(p.T), device=dev).t()
+    built = _kernels.tsit5_kernel(S.lorenz, 3, 1, _kernels.Tsit5Options())
+    outs = rodas5P.launch(built, y0_t, t, p_t, n, dt0=0.0, rtol=RTOL, atol=ATOL,
+                          max_steps=100000, refill=False)
+
+    def run():
+        return rodas5P.launch(built, y0_t, t, p_t, n, dt0=0.0, rtol=RTOL, atol=ATOL,
+                              max_steps=100000, refill=False, outputs=outs)
+
+    o, wall, ev_ms = time_solves(run, 20, 3, torch)
+    att = (o.accepted + o.rejected).float()
+    return {"workload": "Tsit5, Lorenz, 1,024 trajectories, rho ~ U[0,28], y0=(1,0,0), t in [0,1], "
+                        "100 saves, rtol 1e-6 atol 1e-8, fp64",
+            "ms": wall / 20 * 1e3, "kernel_ms": ev_ms,
+            "trajectories_per_s": n / (wall / 20),
+            "attempted_mean": float(att.mean().item()), "attempted_max": int(att.max().item())}
+
+
+def vdp_chain_block(torch, dev, log_fn, n=16384):
+    """C3: Rodas5P, sparse FP32 LU (AMD) on 32 diffusively coupled Van der Pol
+    oscillators (64 vars, open chain), mu ~ LogUniform[10, 1000], y0 ~ U[-2, 2],
+    t in [0, 2000] with 11 saves, rtol 1e-6 atol 1e-8, max_steps 100,000."""
+    from menodax_amd import _kernels, rodas5P, roofline, systems as S
+    from menodax_amd._sparsity import normalize_sparsity
+
+    fn, _ = S.vdp_lattice(32, mu=1.0, d=0.1, wrap=False)
+    mask = normalize_sparsity(S.vdp_sparsity(32, wrap=False), 64)
+    y0, mu = S.vdp_chain_ensemble(32, n)
+    t = torch.tensor(np.linspace(0.0, 2000.0, 11), device=dev)
+    y0_t = torch.tensor(np.ascontiguousarray(y0.T), device=dev).t()
+    p_t = torch.tensor(np.ascontiguousarray(mu.T), device=dev).t()
+    res = {"workload": "Rodas5P sparse (AMD) FP32 LU, 32-oscillator Van der Pol chain (64 vars), "
+                       f"{n} trajectories, mu ~ LogUniform[10,1000], t in [0,2000], 11 saves, "
+                       "rtol 1e-6 atol 1e-8, max_steps 100000"}
+    built = _kernels.rodas5p_kernel(fn, 64, 1, _kernels.KernelOptions(lu_precision="fp32",
+                                                                     sparsity=mask))
+    for refill in (True, False):
+        t0 = time.perf_counter()
+        o = rodas5P.launch(built, y0_t, t, p_t, n, dt0=0.0, rtol=RTOL, atol=ATOL,
+                           max_steps=100000, refill=refill)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        att = (o.accepted + o.rejected).cpu().numpy().astype(np.int64)
+        work = roofline.solve_work(built, int(att.sum()), n * 10)
+        key = "refill" if refill else "plain"
+        res[key] = {"ms": sec * 1e3, "trajectories_per_s": n / sec,
+                    "accepted_steps_per_s": float(o.accepted.sum().item()) / sec,
+                    "roofline_frac": work.roof_seconds() / sec,
+                    "attempted_mean": float(att.mean()), "attempted_max": int(att.max()),
+                    "hit_max_steps": int((o.loop.cpu().numpy() >= 100000).sum())}
+        log_fn(f"vdp chain {key}: {sec*1e3:.1f} ms")
+    return res
+
+
 def adjoint_block(torch, log_fn, n=8192, ckpt_every=64):
     """C4: d(loss)/d(mu, y0) for stiff Van der Pol ensembles."""
     from menodax_amd import rodas5P, systems as S
@@ -273,6 +335,8 @@ def main():
                     help="leave out the 100k uneven stiff ensemble comparison")
     ap.add_argument("--skip-adjoint", action="store_true",
                     help="leave out the adjoint-gradient block")
+    ap.add_argument("--vdp-chain", action="store_true",
+                    help="also run the 64-variable Van der Pol chain ensemble (seconds to minutes)")
     ap.add_argument("--only-uneven-stiff", action="store_true",
                     help="run and print only the uneven stiff ensemble comparison (for profiling)")
     ap.add_argument("--lu", choices=("both", "fp32", "fp64"), default="both",
@@ -493,6 +557,9 @@ def main():
         line["uneven_stiff"] = uneven_stiff(torch, dev, O, log)
     if not args.skip_adjoint:
         line["adjoint"] = adjoint_block(torch, log)
+    line["lorenz_tsit5"] = lorenz_block(torch, dev)
+    if args.vdp_chain:
+        line["vdp_chain"] = vdp_chain_block(torch, dev, log)
 
     print(json.dumps(line), flush=True)
 

@@ -243,3 +243,73 @@ def adjoint_solve(job: AdjointJob, y0_t, p_t):
         _ADJ_FN = _adjoint_function()
     hist, acc, rej, loop = _ADJ_FN.apply(y0_t, p_t, job)
     return hist, (acc, rej, loop)
+
+
+# --- vmap -------------------------------------------------------------------------
+def under_vmap(*xs) -> bool:
+    """Whether any argument is a tensor batched by ``torch.func.vmap``."""
+    import torch
+
+    is_batched = getattr(torch._C._functorch, "is_batchedtensor", None)
+    return is_batched is not None and any(
+        isinstance(x, torch.Tensor) and is_batched(x) for x in xs)
+
+
+def _vmap_function():
+    import torch
+
+    class VmapSolve(torch.autograd.Function):
+        """A solve whose ``torch.func.vmap`` over single-trajectory calls is one
+        ensemble launch (modax/_jax_common.py:49-111): the batched ``y0``
+        and/or ``params`` become the ensemble's trajectories, an unbatched one
+        is broadcast, a batched ``t_span`` is taken as one shared grid (its
+        first row). Each call's result keeps its own leading axis of 1."""
+
+        generate_vmap_rule = False
+
+        @staticmethod
+        def forward(y0, t_span, params, run):
+            return run(y0, t_span, params)
+
+        @staticmethod
+        def setup_context(ctx, inputs, output):
+            pass
+
+        @staticmethod
+        def vmap(info, in_dims, y0, t_span, params, run):
+            b = info.batch_size
+            y_dim, t_dim, p_dim = in_dims[:3]
+
+            def ensemble(x, dim, name):
+                if dim is None:
+                    if x.ndim != 1:
+                        raise NotImplementedError(
+                            f"vmap with an unbatched ensemble-shaped {name} is not supported; "
+                            "call the solver with batched y0/params directly instead")
+                    return x.expand(b, x.shape[0])
+                x = x.movedim(dim, 0)
+                if x.ndim != 2:
+                    raise NotImplementedError(
+                        f"vmap over an already-ensembled {name} is not supported; "
+                        "call the solver with batched y0/params directly instead")
+                return x
+
+            if t_dim is not None:
+                t_span = t_span.movedim(t_dim, 0)[0]
+            out = run(ensemble(y0, y_dim, "y0"), t_span, ensemble(params, p_dim, "params"))
+            # one call's result is (1, n_save, n_vars); stats (1,)
+            return tuple(o[:, None] for o in out), tuple(0 for _ in out)
+
+    return VmapSolve
+
+
+_VMAP_FN = None
+
+
+def vmap_solve(run, y0, t_span, params):
+    """Apply ``run(y0 (N, n), t_span, params (N, n_p)) -> tuple of outputs``
+    through the vmap-aware Function."""
+    global _VMAP_FN
+    if _VMAP_FN is None:
+        _VMAP_FN = _vmap_function()
+    return _VMAP_FN.apply(y0, t_span, params, run)

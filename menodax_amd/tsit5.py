@@ -56,6 +56,23 @@ def solve(
     stream=None,
 ):
     """Integrate an ensemble with Tsit5; see the module docstring."""
+    if _autodiff.under_vmap(y0, t_span, params):
+        # torch.func.vmap over single-trajectory solves: one ensemble launch
+        kwargs = dict(rtol=rtol, atol=atol, first_step=first_step, max_steps=max_steps,
+                      return_stats=True, error_weights=error_weights, pcoeff=pcoeff,
+                      icoeff=icoeff, dcoeff=dcoeff, backend=backend,
+                      trajectories_per_block=trajectories_per_block, refill=refill, stream=stream)
+
+        def run(y, t, p):
+            hist, stats = solve(ode_fn, y, t, p, **kwargs)
+            return (hist, stats["accepted_steps"], stats["rejected_steps"], stats["loop_steps"])
+
+        dev = _common.device()
+        hist, acc, rej, loop = _autodiff.vmap_solve(
+            run, *(_common.as_f64_device(x, dev) for x in (y0, t_span, params)))
+        if return_stats:
+            return hist, _common.solver_stats(acc, rej, loop)
+        return hist
     dev = _common.device()
     y0_t, p_t, n, n_vars = _common.normalize_y0_params(y0, params, dev)
     n_params = p_t.shape[1]

@@ -267,3 +267,19 @@ def test_array_rhs_kernel_compiles_with_the_jacobian_of_ode_fn():
     src, _ = _kernels.rodas5p_source(S.robertson, 3, 3, opts)
     assert "void rhs(" in src and "void neg_jac(" in src
     assert _runtime.rtc_compile(src, "arr.hip")[:4] == b"\x7fELF"
+
+
+def test_vmap_batched_inputs_are_detected():
+    import torch
+    from torch.func import vmap
+
+    from menodax_amd import _autodiff
+
+    seen = []
+
+    def f(x):
+        seen.append(_autodiff.under_vmap(x))
+        return x * 2.0
+
+    vmap(f)(torch.ones(3, 2))
+    assert seen == [True] and not _autodiff.under_vmap(torch.ones(2), np.ones(2))
