@@ -631,10 +631,45 @@ def generate_hook(th: TracedHook) -> str:
     return "\n".join(lines) + "\n"
 
 
-def _with_traceable_globals(fn):
-    """A copy of ``fn`` whose globals see ``min``/``max``/``where`` and the math shim."""
+def _traceable_value(value, depth):
+    """The traceable stand-in for a value a callback reaches through a global
+    or a closure cell: math functions and the math module map to the shim,
+    builtin ``max``/``min`` to their traced forms, and a plain Python helper
+    function is itself rewritten (the pattern of a callback built by a
+    factory that receives its math operations as arguments,
+    examples/dual_backend.py:78-111)."""
+    if value is math:
+        return _MATH_SHIM
+    if value is max:
+        return _sym_max
+    if value is min:
+        return _sym_min
+    if callable(value) and getattr(value, "__module__", None) == "math":
+        name = getattr(value, "__name__", "")
+        if name in _PATCHED_MATH:
+            return getattr(_MATH_SHIM, name)
+    if isinstance(value, types.FunctionType) and depth < 8:
+        return _with_traceable_globals(value, depth + 1)
+    return value
+
+
+def _with_traceable_globals(fn, depth=0):
+    """A copy of ``fn`` whose globals and closure see ``min``/``max``/``where``
+    and the math shim."""
     if not isinstance(fn, types.FunctionType):
         return fn
+    closure = fn.__closure__
+    if closure:
+        cells = []
+        for cell in closure:
+            try:
+                content = cell.cell_contents
+            except ValueError:  # an empty cell
+                cells.append(cell)
+                continue
+            new = _traceable_value(content, depth)
+            cells.append(cell if new is content else types.CellType(new))
+        closure = tuple(cells)
     glb = dict(fn.__globals__)
     glb["max"] = _sym_max
     glb["min"] = _sym_min
@@ -646,7 +681,7 @@ def _with_traceable_globals(fn):
     for name in _PATCHED_MATH:
         if glb.get(name) is getattr(math, name, None):
             glb[name] = getattr(_MATH_SHIM, name)
-    copy = types.FunctionType(fn.__code__, glb, fn.__name__, fn.__defaults__, fn.__closure__)
+    copy = types.FunctionType(fn.__code__, glb, fn.__name__, fn.__defaults__, closure)
     copy.__kwdefaults__ = fn.__kwdefaults__
     return copy
 

@@ -283,3 +283,33 @@ def test_vmap_batched_inputs_are_detected():
 
     vmap(f)(torch.ones(3, 2))
     assert seen == [True] and not _autodiff.under_vmap(torch.ones(2), np.ones(2))
+
+
+def test_callbacks_built_by_a_factory_of_math_operations_trace():
+    """A body written once and closed over its math operations -- the device
+    form of examples/dual_backend.py -- traces like the hand-written callback."""
+    def make_rhs(*, exp, sqrt, maximum, helper):
+        def rhs(y, t, p):
+            return (-exp(-y[0]) * p[0] + helper(y[1]), sqrt(maximum(y[1], 0.25)) - y[0])
+
+        return rhs
+
+    def make_helper(*, exp):
+        def helper(x):
+            return 0.5 * exp(-x * x)
+
+        return helper
+
+    device = make_rhs(exp=math.exp, sqrt=math.sqrt, maximum=max,
+                      helper=make_helper(exp=math.exp))
+
+    def direct(y, t, p):
+        return (-math.exp(-y[0]) * p[0] + 0.5 * math.exp(-y[1] * y[1]),
+                math.sqrt(max(y[1], 0.25)) - y[0])
+
+    a, b = cg.trace(device, 2, 1), cg.trace(direct, 2, 1)
+    rng = np.random.default_rng(0)
+    for _ in range(10):
+        y, p = rng.uniform(-1, 1, 2), rng.uniform(0.5, 2, 1)
+        assert np.array_equal(cg.rhs_from_graph(a, y, 0.0, p), cg.rhs_from_graph(b, y, 0.0, p))
+        np.testing.assert_allclose(cg.rhs_from_graph(a, y, 0.0, p), device(y, 0.0, p), rtol=1e-14)
