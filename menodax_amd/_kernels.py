@@ -107,6 +107,28 @@ def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None
     return defs
 
 
+def tsit5_placement(n_vars: int, block: int, placement=None) -> dict:
+    """The placement macros for a Tsit5 kernel: registers up to
+    ``UNROLLED_MAX_VARS``, above it the rolled stage loop with the seven stage
+    vectors, the stage argument and the state in LDS as far as they fit."""
+    if placement is None:
+        placement = "registers" if n_vars <= UNROLLED_MAX_VARS else "lds"
+    if placement not in PLACEMENTS:
+        raise ValueError(f"placement must be one of {PLACEMENTS}, got {placement!r}")
+    defs = {"MDX_ROLLED": 0 if placement == "registers" else 1,
+            "MDX_LDS_K": 0, "MDX_LDS_U": 0, "MDX_LDS_Y": 0}
+    lds = 0
+    if placement == "lds":
+        for macro, per_lane in (("MDX_LDS_K", 7 * n_vars * 8), ("MDX_LDS_U", 8 * n_vars),
+                                ("MDX_LDS_Y", 8 * n_vars)):
+            if lds + per_lane * block <= LDS_BYTES_PER_CU:
+                defs[macro] = 1
+                lds += per_lane * block
+    defs["lds_bytes"] = lds
+    defs["placement"] = placement
+    return defs
+
+
 @dataclass
 class Built:
     """A compiled kernel pair and what the roofline needs to know about it."""
@@ -325,6 +347,8 @@ class Tsit5Options:
     params_in_regs: bool | None = None
     #: forward sensitivities carried with the state (a ``SensitivitySpec``), or None
     sens: object = None
+    #: "registers", "rolled", "lds" or None (by size); see ``tsit5_placement``
+    placement: str | None = None
 
 
 def tsit5_source(ode_fn, n_vars: int, n_params: int, options: Tsit5Options):
@@ -339,6 +363,7 @@ def tsit5_source(ode_fn, n_vars: int, n_params: int, options: Tsit5Options):
     rhs_src, rhs_fc = _codegen.generate_rhs(traced)
     e1, e2, e3 = pid_exponents(TSIT_EXPONENT, options.pcoeff, options.icoeff, options.dcoeff)
     block = trajectories_per_block_or_default(options.trajectories_per_block)
+    place = tsit5_placement(n_vars, block, options.placement)
     defs = {
         "MDX_N": n_vars,
         "MDX_NP": n_params,
@@ -349,9 +374,13 @@ def tsit5_source(ode_fn, n_vars: int, n_params: int, options: Tsit5Options):
         "MDX_E1": _hex(e1),
         "MDX_E2": _hex(e2),
         "MDX_E3": _hex(e3),
+        "MDX_ROLLED": place["MDX_ROLLED"],
+        "MDX_LDS_K": place["MDX_LDS_K"],
+        "MDX_LDS_U": place["MDX_LDS_U"],
+        "MDX_LDS_Y": place["MDX_LDS_Y"],
     }
     lines = _prelude(defs) + ["namespace sys {", rhs_src, "}  // namespace sys", '#include "tsit5.hpp"']
-    return "\n".join(lines) + "\n", dict(rhs_flops=rhs_fc, block=block)
+    return "\n".join(lines) + "\n", dict(rhs_flops=rhs_fc, block=block, placement=place)
 
 
 @functools.lru_cache(maxsize=None)

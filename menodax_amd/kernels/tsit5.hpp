@@ -8,9 +8,17 @@
 // controller 85-90, loop 238-400): seven stages with first-same-as-last reuse
 // of k7 after an accepted step, the embedded error estimate, the PID
 // controller with factors clamped to [0.2, 10], and the continuous extension
-// at every save time a step passes. The state and its nine vectors live in
-// registers. A save the integration never reaches (max_steps exhausted) is
-// written as NaN; the reference leaves that memory as it found it.
+// at every save time a step passes. A save the integration never reaches
+// (max_steps exhausted) is written as NaN; the reference leaves that memory as
+// it found it.
+//
+// Placement (the reference's "shared"/"local" backends, tsit5.py:92-139):
+//   MDX_ROLLED 0   the step unrolled, the state and its nine vectors in
+//                  registers (small systems);
+//   MDX_ROLLED 1   one stage body run six times; the seven stage vectors (k7
+//                  carried between steps), the stage argument and the state
+//                  in LDS [element][lane] as far as MDX_LDS_K/U/Y say, else in
+//                  scratch. Same arithmetic, same bits.
 #pragma once
 #include "mdx_common.hpp"
 
@@ -19,7 +27,23 @@ namespace t5 {
 
 constexpr int N = MDX_N;
 
-#if MDX_PARAMS_IN_REGS
+#ifndef MDX_ROLLED
+#define MDX_ROLLED 0
+#endif
+#ifndef MDX_LDS_K
+#define MDX_LDS_K 0
+#endif
+#ifndef MDX_LDS_U
+#define MDX_LDS_U 0
+#endif
+#ifndef MDX_LDS_Y
+#define MDX_LDS_Y 0
+#endif
+#if (MDX_LDS_K || MDX_LDS_U || MDX_LDS_Y) && !MDX_ROLLED
+#error "LDS placement needs the rolled stage loop"
+#endif
+
+#if MDX_PARAMS_IN_REGS || MDX_ROLLED
 typedef RegParams<MDX_NP> PA;
 #else
 typedef GlobalParams PA;
@@ -40,9 +64,62 @@ constexpr double E1 = 0.0017800110522257773, E2 = 0.0008164344596567463,
                  E3 = -0.007880878010261994, E4 = 0.1447110071732629, E5 = -0.5823571654525552,
                  E6 = 0.45808210592918686, E7 = -1.0 / 66.0;
 
+#if MDX_ROLLED
+// Stage rows 2..7 (the last is the solution weights B) and abscissae.
+__device__ constexpr double AT[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {A21, 0, 0, 0, 0, 0},
+    {A31, A32, 0, 0, 0, 0},
+    {A41, A42, A43, 0, 0, 0},
+    {A51, A52, A53, A54, 0, 0},
+    {A61, A62, A63, A64, A65, 0},
+    {B1, B2, B3, B4, B5, B6},
+};
+__device__ constexpr double CS[7] = {0.0, C2, C3, C4, C5, C6, C7};
+
+__device__ __forceinline__ unsigned opaque_lane() {
+  unsigned v = threadIdx.x;
+  asm volatile("; lane" : "+v"(v));
+  return v;
+}
+#define MDX_LANE() const unsigned lane_ = opaque_lane()
+// k1..k7 are rows 0..6
+#if MDX_LDS_K
+__shared__ double lds_k[7 * N * MDX_BLOCK];
+#define KQ(st, q, j) lds_k[((q) * N + (j)) * MDX_BLOCK + lane_]
+#else
+#define KQ(st, q, j) (st).kk[q][j]
+#endif
+#if MDX_LDS_Y
+__shared__ double lds_y[N * MDX_BLOCK];
+#define YJ(st, j) lds_y[(j) * MDX_BLOCK + lane_]
+#else
+#define YJ(st, j) (st).y[j]
+#endif
+#if MDX_LDS_U
+__shared__ double lds_u[N * MDX_BLOCK];
+#define UJ(j) lds_u[(j) * MDX_BLOCK + lane_]
+#else
+#define UJ(j) u[j]
+#endif
+struct LdsVec {
+  const double* base;
+  __device__ __forceinline__ double operator[](int j) const { return base[j * MDX_BLOCK]; }
+};
+#endif
+
 struct State {
+#if !MDX_ROLLED
   double y[N];
   double k7[N];  // first-same-as-last carry
+#else
+#if !MDX_LDS_Y
+  double y[N];
+#endif
+#if !MDX_LDS_K
+  double kk[7][N];  // stage vectors; row 6 (k7) is the first-same-as-last carry
+#endif
+#endif
   double t, dt, tf, err_prev, err_prev2;
   int save_idx, n_steps, accepted, rejected;
   bool has_fsal;
@@ -50,12 +127,22 @@ struct State {
 
 __device__ __forceinline__ void init(State& s, const SolveArgs& a, i64 i) {
   const double* __restrict__ times = a.times;
+#if MDX_ROLLED
+  MDX_LANE();
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    YJ(s, j) = a.y0[i * a.y0_ts + (i64)j * a.y0_vs];
+    a.hist[i * a.h_ts + (i64)j * a.h_vs] = YJ(s, j);
+    KQ(s, 6, j) = 0.0;
+  }
+#else
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     s.y[j] = a.y0[i * a.y0_ts + (i64)j * a.y0_vs];
     a.hist[i * a.h_ts + (i64)j * a.h_vs] = s.y[j];
     s.k7[j] = 0.0;
   }
+#endif
   s.t = times[0];
   s.tf = times[a.n_save - 1];
   s.dt = a.dt0 > 0.0 ? a.dt0 : (s.tf - s.t) * 1e-6;
@@ -72,7 +159,145 @@ __device__ __forceinline__ bool active(const State& s, const SolveArgs& a) {
   return s.save_idx < a.n_save && s.t < s.tf && s.n_steps < a.max_steps;
 }
 
-__device__ __forceinline__ void step(State& s, const PA& p, const SolveArgs& a, i64 i) {
+#if MDX_ROLLED
+// The Tsit5 step with the stage loop kept as a loop (large systems). Each stage
+// forms u = y + dt * sum_q a_Sq k_q in the order the unrolled step uses (first
+// term a product, then one fma per stage vector, then the fma with dt and y),
+// so the bits are the same. For scratch placement, the never-executed stores
+// (max_steps >= 0 is checked before launch) keep the arrays in memory.
+__device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i) {
+  double dt_use = s.dt;
+  if (dt_use > s.tf - s.t) dt_use = s.tf - s.t;
+  if (dt_use < 1e-30) dt_use = 1e-30;
+  const double t = s.t;
+  MDX_LANE();
+#if !MDX_LDS_U
+  double u[N];
+#endif
+  {
+    double out[N];
+    if (s.has_fsal) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) out[j] = KQ(s, 6, j);
+    } else {
+#if MDX_LDS_Y
+      sys::rhs(LdsVec{&lds_y[lane_]}, t, p, out);
+#else
+      sys::rhs(s.y, t, p, out);
+#endif
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) KQ(s, 0, j) = out[j];
+  }
+#define MDX_PIN(arr, len) \
+  if (a.max_steps < 0) arr[(a.n_save + S) % (len)] = 0
+#pragma unroll 1
+  for (int S = 1; S < 7; ++S) {
+    MDX_LANE();
+#if !MDX_LDS_K
+    MDX_PIN(s.kk[S], N);
+#endif
+#if !MDX_LDS_Y
+    MDX_PIN(s.y, N);
+#endif
+#if !MDX_PARAMS_IN_REGS && MDX_NP > 0
+    MDX_PIN(p.v, MDX_NP);
+#endif
+    const double* __restrict__ ar = AT[S];
+    double q[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) q[j] = ar[0] * KQ(s, 0, j);
+#pragma unroll 1
+    for (int r = 1; r < S; ++r) {
+      const double c = ar[r];
+#pragma unroll
+      for (int j = 0; j < N; ++j) q[j] = fmaT(c, KQ(s, r, j), q[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) UJ(j) = fmaT(dt_use, q[j], YJ(s, j));
+    double out[N];
+#if MDX_LDS_U
+    sys::rhs(LdsVec{&lds_u[lane_]}, t + CS[S] * dt_use, p, out);
+#else
+    sys::rhs(u, t + CS[S] * dt_use, p, out);
+#endif
+#pragma unroll
+    for (int j = 0; j < N; ++j) KQ(s, S, j) = out[j];
+  }
+#undef MDX_PIN
+
+  double err = 0.0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double e = E1 * KQ(s, 0, j);
+    e = fmaT(E2, KQ(s, 1, j), e);
+    e = fmaT(E3, KQ(s, 2, j), e);
+    e = fmaT(E4, KQ(s, 3, j), e);
+    e = fmaT(E5, KQ(s, 4, j), e);
+    e = fmaT(E6, KQ(s, 5, j), e);
+    e = fmaT(E7, KQ(s, 6, j), e);
+    const double err_est = dt_use * e;
+    const double ya = absT(YJ(s, j)), yb = absT(UJ(j));
+    const double scale = fmaT(a.rtol, ya > yb ? ya : yb, a.atol);
+    const double r = weight(a, i, j) * err_est / scale;
+    err = fmaT(r, r, err);
+  }
+  const double err_norm = __builtin_sqrt(err / (double)MDX_N_ERROR);
+  const bool accept = err_norm <= 1.0 && !__builtin_isnan(err_norm);
+
+  double t_new = t;
+  if (accept) {
+    t_new = t + dt_use;
+    const double* __restrict__ times = a.times;
+    while (s.save_idx < a.n_save) {
+      const double ts = times[s.save_idx];
+      const double at = absT(ts);
+      if (!(ts <= t_new + 1e-12 * (1.0 > at ? 1.0 : at))) break;
+      const double th = (ts - t) / dt_use;
+      const double b1 = -1.0530884977290216 * th * (th - 1.3299890189751412) *
+                        (th * th - 1.4364028541716351 * th + 0.7139816917074209);
+      const double b2 =
+          0.1017 * th * th * (th * th - 2.1966568338249754 * th + 1.2949852507374631);
+      const double b3 = 2.490627285651252793 * th * th *
+                        (th * th - 2.38535645472061657 * th + 1.57803468208092486);
+      const double b4 = -16.54810288924490272 * (th - 1.21712927295533244) *
+                        (th - 0.61620406037800089) * th * th;
+      const double b5 = 47.37952196281928122 * (th - 1.203071208372362603) *
+                        (th - 0.658047292653547382) * th * th;
+      const double b6 = -34.87065786149660974 * (th - 1.2) * (th - 0.666666666666666667) * th * th;
+      const double b7 = 2.5 * (th - 1.0) * (th - 0.6) * th * th;
+      double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        double q = b1 * KQ(s, 0, j);
+        q = fmaT(b2, KQ(s, 1, j), q);
+        q = fmaT(b3, KQ(s, 2, j), q);
+        q = fmaT(b4, KQ(s, 3, j), q);
+        q = fmaT(b5, KQ(s, 4, j), q);
+        q = fmaT(b6, KQ(s, 5, j), q);
+        q = fmaT(b7, KQ(s, 6, j), q);
+        row[(i64)j * a.h_vs] = fmaT(dt_use, q, YJ(s, j));
+      }
+      s.save_idx++;
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) YJ(s, j) = UJ(j);
+    s.accepted++;
+    s.has_fsal = true;
+  } else {
+    s.rejected++;
+#pragma unroll
+    for (int j = 0; j < N; ++j) KQ(s, 6, j) = 0.0;
+    s.has_fsal = false;
+  }
+  const double factor_ =
+      pid_factor(err_norm, accept, s.err_prev, s.err_prev2, MDX_E1, MDX_E2, MDX_E3, 0.2, 10.0);
+  s.dt = dt_use * factor_;
+  s.t = t_new;
+  s.n_steps++;
+}
+#else
+__device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i) {
   double dt_use = s.dt;
   if (dt_use > s.tf - s.t) dt_use = s.tf - s.t;
   if (dt_use < 1e-30) dt_use = 1e-30;
@@ -200,6 +425,7 @@ __device__ __forceinline__ void step(State& s, const PA& p, const SolveArgs& a, 
   s.t = t_new;
   s.n_steps++;
 }
+#endif
 
 __device__ __forceinline__ void finish(State& s, const SolveArgs& a, i64 i) {
   const double nan = __builtin_nan("");

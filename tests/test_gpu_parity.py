@@ -325,6 +325,30 @@ def test_tsit5_vdp_ring_twelve_steps():
     assert (st["loop_steps"] == 12).all() and (st["rejected_steps"] == 0).all()
 
 
+@pytest.mark.parametrize("placement", ["registers", "rolled", "lds"])
+def test_tsit5_placements_are_bit_exact(placement):
+    """The reference's shared/local backends as placements: registers, rolled
+    through scratch, rolled through LDS -- the same bits as the oracle."""
+    from menodax_amd import _kernels, rodas5P as R5
+
+    fn, _ = S.vdp_lattice(12, mu=1.0)
+    rng = np.random.default_rng(9)
+    y0 = rng.uniform(-2, 2, size=(300, 24))
+    p = rng.uniform(0.5, 3.0, size=(300, 1))
+    t = np.linspace(0.0, 2.0, 9)
+    ref, rst = O.tsit5(O.vdp_lattice(12, mu=1.0), y0, t, p, rtol=1e-7, atol=1e-9)
+    built = _kernels.tsit5_kernel(fn, 24, 1, _kernels.Tsit5Options(placement=placement))
+    import torch
+
+    dev = torch.device("cuda")
+    tt = torch.tensor(t, device=dev)
+    for refill in (False, True):
+        out = R5.launch(built, torch.tensor(y0, device=dev), tt, torch.tensor(p, device=dev), 300,
+                        dt0=0.0, rtol=1e-7, atol=1e-9, max_steps=100000, refill=refill)
+        hist, st = R5.finish(out, True, True)
+        _same(hist, st, ref, rst)
+
+
 def test_tsit5_backend_names():
     fn, _ = S.vdp_lattice(16, mu=1.0)
     y0, p = S.vdp_scenario(16, 8, 42, divergence=1.0)
