@@ -1,12 +1,14 @@
 """Tsit5 on MI355X: ``solve(ode_fn, y0, t_span, params, ...)``.
 
 The API is the reference's (modax/tsit5.py:623-709). ``backend`` keeps the
-reference's values: there, "shared" put the nine per-trajectory vectors in
-shared memory and "local" in thread-local arrays (tsit5.py:92-139). Here the
-vectors live in VGPRs in both cases -- 9 n doubles per lane fits the register
-file for every system the shared backend accepted -- so the two names select
-the same kernel and are bit-identical by construction; the reference's size
-check on "shared" is kept so the same calls fail the same way.
+reference's values and size checks: there, "shared" put the nine
+per-trajectory vectors in shared memory and "local" in thread-local arrays
+(tsit5.py:92-139). Here where the vectors live is a placement chosen by size
+(``_kernels.tsit5_placement``): in VGPRs up to 16 variables -- which covers
+every system the shared backend accepted, so both names select the same kernel
+there -- and above that a rolled stage loop with the stage vectors, stage
+argument and state in LDS as far as they fit. Placements are bit-identical.
+Differentiation and ``torch.func.vmap`` work as for ``rodas5P.solve``.
 """
 
 from __future__ import annotations
