@@ -109,12 +109,18 @@ def endpoint_error(hist_gpu, y0, params, times, sample, oracle, O, lu):
             "units": "multiples of atol + rtol*|y_true| at t_end"}
 
 
-def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64):
+UNEVEN_VARIANTS = ("fp32_refill", "fp32_plain", "fp64_refill", "fp64_plain")
+
+
+def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64, variants=UNEVEN_VARIANTS):
     """The uneven stiff ensemble: dy/dt = A y + 0.01 y*y, 50 variables, A block
     arrow (2x2 rotation blocks plus a dense hub row), eigenvalue real parts
     -LogUniform down to a per-trajectory stiffness ceiling up to 1e5, t in
     [0, 10] with 20 saves, rtol 1e-6 atol 1e-8. Times the FP32-LU refill
-    kernel and the plain FP64 kernel (and the other two combinations)."""
+    kernel and the plain FP64 kernel (and the other two combinations);
+    ``variants`` narrows that to some of them (profiling runs), which leaves
+    out the comparison and the endpoint error unless both headline variants
+    are among them."""
     from menodax_amd import _kernels, _sparse_direct as SD, rodas5P, roofline, systems as S
     from menodax_amd._sparsity import normalize_sparsity
 
@@ -133,6 +139,9 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64):
         built = _kernels.rodas5p_kernel(fn, n_vars, n_p, _kernels.KernelOptions(lu_precision=lu,
                                                                                 sparsity=sp))
         for refill in (True, False):
+            key = f"{lu}_{'refill' if refill else 'plain'}"
+            if key not in variants:
+                continue
             outs = rodas5P.launch(built, y0_t, t_t, p_t, n, dt0=0.0, rtol=RTOL, atol=ATOL,
                                   max_steps=100000, refill=refill)
             ws = outs._keep[-1]
@@ -145,17 +154,20 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64):
             ms = wall / 2 * 1e3
             att = (o.accepted + o.rejected).cpu().numpy().astype(np.int64)
             work = roofline.solve_work(built, int(att.sum()), n * (len(times) - 1))
-            key = f"{lu}_{'refill' if refill else 'plain'}"
             res[key] = {"ms": ms, "trajectories_per_s": n / (ms / 1e3),
                         "accepted_steps_per_s": float(o.accepted.sum().item()) / (ms / 1e3),
                         "roofline_frac": work.roof_seconds() / (ev_ms / 1e3),
                         "placement": built.extra.get("placement", {}).get("placement")}
             hists[key] = o.ensemble_view().cpu().numpy()
             log_fn(f"uneven stiff {key}: {ms:.1f} ms")
+    if not {"fp32_refill", "fp64_plain"} <= set(res):
+        return {"variants": res}
     steps = att
     edges = np.logspace(np.log10(max(1, steps.min())), np.log10(steps.max() + 1), 9)
     counts, _ = np.histogram(steps, bins=edges)
-    same = all(np.array_equal(hists[f"{lu}_refill"], hists[f"{lu}_plain"]) for lu in ("fp32", "fp64"))
+    pairs = [lu for lu in ("fp32", "fp64") if f"{lu}_refill" in hists and f"{lu}_plain" in hists]
+    same = (all(np.array_equal(hists[f"{lu}_refill"], hists[f"{lu}_plain"]) for lu in pairs)
+            if pairs else None)
     # endpoint error on a sample against the oracle: FP64 LU at the same
     # tolerance, and a tight FP64 solve
     idx = np.arange(sample)
@@ -176,7 +188,7 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64):
                     "rtol 1e-6 atol 1e-8, fp64 state, sparse LU (AMD)",
         **res,
         "fp32_refill_over_fp64_plain": res["fp64_plain"]["ms"] / res["fp32_refill"]["ms"],
-        "refill_bit_identical_to_plain": bool(same),
+        "refill_bit_identical_to_plain": same,
         "attempted_steps": {"p1": float(np.percentile(steps, 1)),
                             "median": float(np.median(steps)),
                             "p99": float(np.percentile(steps, 99)), "max": int(steps.max()),
@@ -339,6 +351,9 @@ def main():
                     help="also run the 64-variable Van der Pol chain ensemble (seconds to minutes)")
     ap.add_argument("--only-uneven-stiff", action="store_true",
                     help="run and print only the uneven stiff ensemble comparison (for profiling)")
+    ap.add_argument("--uneven-variants", default=",".join(UNEVEN_VARIANTS),
+                    help="with --only-uneven-stiff: comma-separated subset of "
+                         + ", ".join(UNEVEN_VARIANTS) + " (profiling runs)")
     ap.add_argument("--lu", choices=("both", "fp32", "fp64"), default="both",
                     help="profile runs: time one LU precision only (the JSON line needs both)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -360,8 +375,12 @@ def main():
     if args.only_uneven_stiff:
         from oracle import oracle as O
 
-        print(json.dumps({"uneven_stiff": uneven_stiff(torch, torch.device("cuda", 0), O, log)}),
-              flush=True)
+        variants = tuple(v for v in args.uneven_variants.split(",") if v)
+        unknown = set(variants) - set(UNEVEN_VARIANTS)
+        if unknown:
+            ap.error(f"unknown --uneven-variants {sorted(unknown)}")
+        print(json.dumps({"uneven_stiff": uneven_stiff(torch, torch.device("cuda", 0), O, log,
+                                                       variants=variants)}), flush=True)
         return
 
     dev = torch.device("cuda", 0)

@@ -285,9 +285,10 @@ def _solve_source(layout, unrolled: bool) -> str:
     row_ptr, diag_ptr = layout.row_ptr, layout.diag_ptr
     origin, col_origin = layout.row_origin, layout.col_origin
     if unrolled:
-        lines = ["template <class T>",
-                 "__device__ __forceinline__ void sparse_solve(const T* __restrict__ lu, "
-                 "T* __restrict__ r) {"]
+        # any indexable factorisation: a register array, or an accessor over
+        # registers the caller chose (accumulation registers)
+        lines = ["template <class T, class L>",
+                 "__device__ __forceinline__ void sparse_solve(const L& lu, T* __restrict__ r) {"]
         for i in range(n):
             row = int(origin[i])
             a, b = int(row_ptr[i]), int(diag_ptr[i])

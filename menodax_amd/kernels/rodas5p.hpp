@@ -124,6 +124,11 @@ typedef GlobalParams PA;
 #ifndef MDX_DT_ZERO
 #define MDX_DT_ZERO 0
 #endif
+// Elements per chunk where a loop over the per-lane vectors loads a chunk
+// before computing with it.
+#ifndef MDX_CHUNK
+#define MDX_CHUNK 8
+#endif
 #if (MDX_LDS_K || MDX_LDS_Y || MDX_LDS_U) && !MDX_ROLLED
 #error "LDS placement needs the rolled stage loop"
 #endif
@@ -451,8 +456,8 @@ __device__ __forceinline__ void factor(LUT* __restrict__ lu, Pivots& ipiv) {
 #endif
 }
 
-__device__ __forceinline__ void solve(const LUT* __restrict__ lu, const Pivots& ipiv,
-                                      LUT* __restrict__ r) {
+template <class L>
+__device__ __forceinline__ void solve(const L& lu, const Pivots& ipiv, LUT* __restrict__ r) {
 #if MDX_SPARSE
   (void)ipiv;
   sys::sparse_solve(lu, r);
@@ -460,6 +465,37 @@ __device__ __forceinline__ void solve(const LUT* __restrict__ lu, const Pivots& 
   dense_solve(lu, ipiv, r);
 #endif
 }
+
+#ifndef MDX_LU_AGPR
+#define MDX_LU_AGPR 0
+#endif
+#if MDX_LU_AGPR
+#if !MDX_LU_FP32 || !MDX_SPARSE || !MDX_ROLLED || MDX_ADJOINT
+#error "the factorisation goes to accumulation registers only for the rolled FP32 sparse step"
+#endif
+// The rolled step's FP32 sparse factorisation, held in the accumulation
+// registers between its eight solves. Otherwise it would occupy most of the
+// architectural registers through the whole stage loop, and the stage sums
+// would have no registers left to keep several LDS loads in flight. Each
+// element is one v_accvgpr_write after the factorisation and one
+// v_accvgpr_read per solve; a read takes the stage index as an operand it
+// does not use, so it is not hoisted out of the stage loop.
+struct AccLU {
+  float a[MDX_NLU];
+  __device__ __forceinline__ void set(int q, float x) {
+    asm("v_accvgpr_write_b32 %0, %1" : "=a"(a[q]) : "v"(x));
+  }
+};
+struct AccLUView {
+  const AccLU& lu;
+  int stage;
+  __device__ __forceinline__ float operator[](int q) const {
+    float x;
+    asm("v_accvgpr_read_b32 %0, %1 ; stage %2" : "=v"(x) : "a"(lu.a[q]), "s"(stage));
+    return x;
+  }
+};
+#endif
 
 #if MDX_ADJOINT
 // M^T x = b against the step's factorisation, in double (the adjoint).
@@ -506,10 +542,9 @@ __device__ __forceinline__ void sens_time_derivative(const Z& y, double t, const
 // corrected by L_d k_y, one second-order sweep at the step's base point
 // (rodas5P.py:512-561). ``sv`` holds the directions' stage values (double),
 // ``ks`` stores into the stage vectors.
-template <class Z, class KS>
+template <class Z, class KS, class L>
 __device__ __forceinline__ void sens_stage(int S, const Z& ybase, double t, const PA& p,
-                                           const double* __restrict__ sv,
-                                           const LUT* __restrict__ lu,
+                                           const double* __restrict__ sv, const L& lu,
                                            const Pivots& ipiv, KS ks) {
 #if MDX_NSENS > 0
   double ky[N];
@@ -627,6 +662,29 @@ __device__ __forceinline__ bool active(const State& s, const SolveArgs& a) {
   return s.save_idx < a.n_save && s.t < s.tf && s.n_steps < a.max_steps;
 }
 
+// acc[jj] = sum_{q<S} c_q k_q[j0+jj] over one chunk of elements, q ascending
+// (the rolled step's stage sums). A chunk's loads are independent, so they are
+// all in flight before the first is waited for; summing every element at once
+// would keep one accumulator per element live and leave the loads no registers.
+template <class KS>
+__device__ __forceinline__ void chunk_sum(const double* __restrict__ c, int S, int j0, KS ks,
+                                          double* __restrict__ acc) {
+#pragma unroll
+  for (int jj = 0; jj < MDX_CHUNK; ++jj)
+    if (j0 + jj < NA) acc[jj] = c[0] * (double)ks(0, j0 + jj);
+#pragma unroll 1
+  for (int q = 1; q < S; ++q) {
+    const double cq = c[q];
+    LUT kv[MDX_CHUNK];
+#pragma unroll
+    for (int jj = 0; jj < MDX_CHUNK; ++jj)
+      if (j0 + jj < NA) kv[jj] = ks(q, j0 + jj);
+#pragma unroll
+    for (int jj = 0; jj < MDX_CHUNK; ++jj)
+      if (j0 + jj < NA) acc[jj] = fmaT(cq, (double)kv[jj], acc[jj]);
+  }
+}
+
 // One attempted step, accepted or rejected, with the saves it passes.
 __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i) {
 #if MDX_ROLLED && !MDX_PARAMS_IN_REGS && MDX_NP > 0
@@ -659,6 +717,11 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
   for (int d = 0; d < N; ++d) lu[diag_slot(d)] = lu[diag_slot(d)] + (LUT)dtgamma_inv;
   Pivots ipiv;
   factor(lu, ipiv);
+#if MDX_LU_AGPR
+  AccLU lu_acc;
+#pragma unroll
+  for (int q = 0; q < MDX_NLU; ++q) lu_acc.set(q, lu[q]);
+#endif
 
 #if !MDX_LDS_K
   LUT k[8][NA];
@@ -684,6 +747,8 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
 #pragma unroll 1
   for (int S = 0; S < 8; ++S) {
     MDX_LANE();
+    // the stage vectors through this stage's lane register
+    auto kst = [&](int q, int j) -> LUT { return KQ(q, j); };
 #if !MDX_LDS_K
     MDX_PIN(k[S & 7], NA);
 #endif
@@ -698,78 +763,108 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
 #endif
     double ts;
     if (S == 0) {
+#if !(MDX_LDS_U && MDX_LDS_Y)
 #pragma unroll
-      for (int j = 0; j < NA; ++j) UJ(j) = YJ(s, j);
+      for (int j0 = 0; j0 < NA; j0 += MDX_CHUNK) {
+        double yv[MDX_CHUNK];
+#pragma unroll
+        for (int jj = 0; jj < MDX_CHUNK; ++jj)
+          if (j0 + jj < NA) yv[jj] = YJ(s, j0 + jj);
+#pragma unroll
+        for (int jj = 0; jj < MDX_CHUNK; ++jj)
+          if (j0 + jj < NA) UJ(j0 + jj) = yv[jj];
+      }
+#endif
       ts = t;
     } else if (S < 6) {
-      // u = y + sum_q A_{S,q} k_q (rodas5P.py:649-661)
+      // u = y + sum_q A_{S,q} k_q (rodas5P.py:649-661), a chunk of elements at
+      // a time (see chunk_sum)
       const double* __restrict__ am = AM[S - 1];
-      double acc[NA];
 #pragma unroll
-      for (int j = 0; j < NA; ++j) acc[j] = am[0] * (double)KQ(0, j);
-#pragma unroll 1
-      for (int q = 1; q < S; ++q) {
-        const double c = am[q];
+      for (int j0 = 0; j0 < NA; j0 += MDX_CHUNK) {
+        double acc[MDX_CHUNK], yv[MDX_CHUNK];
+        chunk_sum(am, S, j0, kst, acc);
 #pragma unroll
-        for (int j = 0; j < NA; ++j) acc[j] = fmaT(c, (double)KQ(q, j), acc[j]);
+        for (int jj = 0; jj < MDX_CHUNK; ++jj)
+          if (j0 + jj < NA) yv[jj] = YJ(s, j0 + jj);
+#pragma unroll
+        for (int jj = 0; jj < MDX_CHUNK; ++jj)
+          if (j0 + jj < NA) UJ(j0 + jj) = yv[jj] + acc[jj];
       }
-#pragma unroll
-      for (int j = 0; j < NA; ++j) UJ(j) = YJ(s, j) + acc[j];
       ts = S < 5 ? t + CT[S - 1] * dt_use : t_end;
     } else {
 #pragma unroll
-      for (int j = 0; j < NA; ++j) UJ(j) = UJ(j) + (double)KQ(S - 1, j);
+      for (int j0 = 0; j0 < NA; j0 += MDX_CHUNK) {
+        double uv[MDX_CHUNK];
+        LUT kv[MDX_CHUNK];
+#pragma unroll
+        for (int jj = 0; jj < MDX_CHUNK; ++jj)
+          if (j0 + jj < NA) uv[jj] = UJ(j0 + jj), kv[jj] = KQ(S - 1, j0 + jj);
+#pragma unroll
+        for (int jj = 0; jj < MDX_CHUNK; ++jj)
+          if (j0 + jj < NA) UJ(j0 + jj) = uv[jj] + (double)kv[jj];
+      }
       ts = t_end;
     }
-    // sum_q C_Sq k_qj, formed before the right-hand side so that the two do
-    // not hold registers at the same time
-    double acc[NA];
-    if (S > 0) {
-      const double* __restrict__ cm = CM[S];
-#pragma unroll
-      for (int j = 0; j < NA; ++j) acc[j] = cm[0] * (double)KQ(0, j);
-#pragma unroll 1
-      for (int q = 1; q < S; ++q) {
-        const double c = cm[q];
-#pragma unroll
-        for (int j = 0; j < NA; ++j) acc[j] = fmaT(c, (double)KQ(q, j), acc[j]);
-      }
-    }
     double work[NA];
-#if MDX_LDS_U
+#if MDX_LDS_U && MDX_LDS_Y
+    // the first stage evaluates at the state itself, read where it lies
+    rhs_aug(LdsVec{S == 0 ? &lds_y[lane_] : &lds_u[lane_]}, ts, p, work);
+#elif MDX_LDS_U
     rhs_aug(LdsVec{&lds_u[lane_]}, ts, p, work);
 #else
     rhs_aug(u, ts, p, work);
 #endif
-    // rhs_j = LU(work_j + dt*d_S*dT_j + (sum_q C_Sq k_qj) / dt) (rodas5P.py:611-647)
-    // one branch per stage, outside the element loop, so each element is one
-    // formula rather than all three and a select
+    // rhs_j = LU(work_j + dt*d_S*dT_j + (sum_q C_Sq k_qj) / dt) (rodas5P.py:611-647),
+    // the sum formed after the right-hand side, a chunk at a time, so that it
+    // never holds a register for every element; one branch per stage and
+    // chunk, outside the element loop, so each element is one formula rather
+    // than all three and a select
     LUT r[N];
-    if (S == 0) {
-      const double dd = dt_use * DV[0];
+    const double* __restrict__ cm = CM[S];
 #pragma unroll
-      for (int j = 0; j < NA; ++j) {
-        const double v = fmaT(dd, DTJ(j), work[j]);
-        if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
-      }
-    } else if (S < 5) {
-      const double dd = dt_use * DV[S];
+    for (int j0 = 0; j0 < NA; j0 += MDX_CHUNK) {
+      if (S == 0) {
+        const double dd = dt_use * DV[0];
 #pragma unroll
-      for (int j = 0; j < NA; ++j) {
-        const double v = fmaT(acc[j], inv_dt, fmaT(dd, DTJ(j), work[j]));
-        if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
-      }
-    } else {
+        for (int jj = 0; jj < MDX_CHUNK; ++jj) {
+          const int j = j0 + jj;
+          if (j >= NA) break;
+          const double v = fmaT(dd, DTJ(j), work[j]);
+          if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
+        }
+      } else {
+        double acc[MDX_CHUNK];
+        chunk_sum(cm, S, j0, kst, acc);
+        if (S < 5) {
+          const double dd = dt_use * DV[S];
 #pragma unroll
-      for (int j = 0; j < NA; ++j) {
-        const double v = fmaT(acc[j], inv_dt, work[j]);
-        if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
+          for (int jj = 0; jj < MDX_CHUNK; ++jj) {
+            const int j = j0 + jj;
+            if (j >= NA) break;
+            const double v = fmaT(acc[jj], inv_dt, fmaT(dd, DTJ(j), work[j]));
+            if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
+          }
+        } else {
+#pragma unroll
+          for (int jj = 0; jj < MDX_CHUNK; ++jj) {
+            const int j = j0 + jj;
+            if (j >= NA) break;
+            const double v = fmaT(acc[jj], inv_dt, work[j]);
+            if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
+          }
+        }
       }
     }
-    solve(lu, ipiv, r);
+#if MDX_LU_AGPR
+    const AccLUView lu_s{lu_acc, S};
+#else
+    const LUT (&lu_s)[MDX_NLU] = lu;
+#endif
+    solve(lu_s, ipiv, r);
 #pragma unroll
     for (int j = 0; j < N; ++j) KQ(S, j) = r[j];
-    sens_stage(S, ybase, t, p, sv, lu, ipiv, ks);
+    sens_stage(S, ybase, t, p, sv, lu_s, ipiv, ks);
   }
 #undef MDX_PIN
 #else
@@ -810,15 +905,30 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
 #endif
 
   // Weighted RMS error of the embedded estimate k7 (rodas5P.py:903-911).
+  // Loads a chunk at a time ahead of the arithmetic, so that several are in
+  // flight when the vectors live in LDS or scratch.
   double err = 0.0;
 #pragma unroll
-  for (int j = 0; j < NA; ++j) {
-    const double k7 = (double)KQ(7, j);
-    const double y_new = UJ(j) + k7;
-    const double ya = absT(YJ(s, j)), yb = absT(y_new);
-    const double scale = fmaT(a.rtol, ya > yb ? ya : yb, a.atol);
-    const double r = weight(a, i, j) * k7 / scale;
-    err = fmaT(r, r, err);
+  for (int j0 = 0; j0 < NA; j0 += MDX_CHUNK) {
+    double k7v[MDX_CHUNK], uv[MDX_CHUNK], yv[MDX_CHUNK];
+#pragma unroll
+    for (int jj = 0; jj < MDX_CHUNK; ++jj)
+      if (j0 + jj < NA) {
+        k7v[jj] = (double)KQ(7, j0 + jj);
+        uv[jj] = UJ(j0 + jj);
+        yv[jj] = YJ(s, j0 + jj);
+      }
+#pragma unroll
+    for (int jj = 0; jj < MDX_CHUNK; ++jj) {
+      const int j = j0 + jj;
+      if (j >= NA) break;
+      const double k7 = k7v[jj];
+      const double y_new = uv[jj] + k7;
+      const double ya = absT(yv[jj]), yb = absT(y_new);
+      const double scale = fmaT(a.rtol, ya > yb ? ya : yb, a.atol);
+      const double r = weight(a, i, j) * k7 / scale;
+      err = fmaT(r, r, err);
+    }
   }
   const double err_norm = __builtin_sqrt(err / (double)MDX_N_ERROR);
   const bool accept = err_norm <= 1.0 && !__builtin_isnan(err_norm);
@@ -872,7 +982,16 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
       s.save_idx++;
     }
 #pragma unroll
-    for (int j = 0; j < NA; ++j) YJ(s, j) = UJ(j) + (double)KQ(7, j);
+    for (int j0 = 0; j0 < NA; j0 += MDX_CHUNK) {
+      double uv[MDX_CHUNK];
+      LUT kv[MDX_CHUNK];
+#pragma unroll
+      for (int jj = 0; jj < MDX_CHUNK; ++jj)
+        if (j0 + jj < NA) uv[jj] = UJ(j0 + jj), kv[jj] = KQ(7, j0 + jj);
+#pragma unroll
+      for (int jj = 0; jj < MDX_CHUNK; ++jj)
+        if (j0 + jj < NA) YJ(s, j0 + jj) = uv[jj] + (double)kv[jj];
+    }
     s.t = t + dt_use;
     s.accepted++;
   } else {
