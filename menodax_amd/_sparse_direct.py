@@ -285,8 +285,7 @@ def _solve_source(layout, unrolled: bool) -> str:
     row_ptr, diag_ptr = layout.row_ptr, layout.diag_ptr
     origin, col_origin = layout.row_origin, layout.col_origin
     if unrolled:
-        # any indexable factorisation: a register array, or an accessor over
-        # registers the caller chose (accumulation registers)
+        # the factorisation as any indexable type (a register array here)
         lines = ["template <class T, class L>",
                  "__device__ __forceinline__ void sparse_solve(const L& lu, T* __restrict__ r) {"]
         for i in range(n):

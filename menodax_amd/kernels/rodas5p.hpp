@@ -127,7 +127,7 @@ typedef GlobalParams PA;
 // Elements per chunk where a loop over the per-lane vectors loads a chunk
 // before computing with it.
 #ifndef MDX_CHUNK
-#define MDX_CHUNK 8
+#define MDX_CHUNK 16
 #endif
 #if (MDX_LDS_K || MDX_LDS_Y || MDX_LDS_U) && !MDX_ROLLED
 #error "LDS placement needs the rolled stage loop"
@@ -466,37 +466,6 @@ __device__ __forceinline__ void solve(const L& lu, const Pivots& ipiv, LUT* __re
 #endif
 }
 
-#ifndef MDX_LU_AGPR
-#define MDX_LU_AGPR 0
-#endif
-#if MDX_LU_AGPR
-#if !MDX_LU_FP32 || !MDX_SPARSE || !MDX_ROLLED || MDX_ADJOINT
-#error "the factorisation goes to accumulation registers only for the rolled FP32 sparse step"
-#endif
-// The rolled step's FP32 sparse factorisation, held in the accumulation
-// registers between its eight solves. Otherwise it would occupy most of the
-// architectural registers through the whole stage loop, and the stage sums
-// would have no registers left to keep several LDS loads in flight. Each
-// element is one v_accvgpr_write after the factorisation and one
-// v_accvgpr_read per solve; a read takes the stage index as an operand it
-// does not use, so it is not hoisted out of the stage loop.
-struct AccLU {
-  float a[MDX_NLU];
-  __device__ __forceinline__ void set(int q, float x) {
-    asm("v_accvgpr_write_b32 %0, %1" : "=a"(a[q]) : "v"(x));
-  }
-};
-struct AccLUView {
-  const AccLU& lu;
-  int stage;
-  __device__ __forceinline__ float operator[](int q) const {
-    float x;
-    asm("v_accvgpr_read_b32 %0, %1 ; stage %2" : "=v"(x) : "a"(lu.a[q]), "s"(stage));
-    return x;
-  }
-};
-#endif
-
 #if MDX_ADJOINT
 // M^T x = b against the step's factorisation, in double (the adjoint).
 __device__ __forceinline__ void solve_t(const double* __restrict__ lu, const Pivots& ipiv,
@@ -717,11 +686,6 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
   for (int d = 0; d < N; ++d) lu[diag_slot(d)] = lu[diag_slot(d)] + (LUT)dtgamma_inv;
   Pivots ipiv;
   factor(lu, ipiv);
-#if MDX_LU_AGPR
-  AccLU lu_acc;
-#pragma unroll
-  for (int q = 0; q < MDX_NLU; ++q) lu_acc.set(q, lu[q]);
-#endif
 
 #if !MDX_LDS_K
   LUT k[8][NA];
@@ -856,15 +820,10 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
         }
       }
     }
-#if MDX_LU_AGPR
-    const AccLUView lu_s{lu_acc, S};
-#else
-    const LUT (&lu_s)[MDX_NLU] = lu;
-#endif
-    solve(lu_s, ipiv, r);
+    solve(lu, ipiv, r);
 #pragma unroll
     for (int j = 0; j < N; ++j) KQ(S, j) = r[j];
-    sens_stage(S, ybase, t, p, sv, lu_s, ipiv, ks);
+    sens_stage(S, ybase, t, p, sv, lu, ipiv, ks);
   }
 #undef MDX_PIN
 #else
