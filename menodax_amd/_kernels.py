@@ -22,6 +22,9 @@ DEFAULT_BLOCK = 64
 PARAMS_IN_REGS_MAX = 16
 #: systems up to this size run the fully unrolled, register-resident step
 UNROLLED_MAX_VARS = 16
+#: the adjoint kernel holds eight stage vectors, their adjoints and the
+#: factorisation in both precisions; unrolled only up to this many variables
+ADJOINT_UNROLLED_MAX_VARS = 8
 #: LDS per compute unit on gfx950
 LDS_BYTES_PER_CU = 160 * 1024
 PLACEMENTS = ("registers", "rolled", "lds")
@@ -206,8 +209,10 @@ def dense_lu_flops(n):
 
 def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, adjoint=False):
     """Generate the full Rodas5P kernel source; returns (source, details).
-    ``adjoint=True`` generates the discrete-adjoint kernel instead (rolled
-    step, scratch placement, reverse-mode contractions of the callback)."""
+    ``adjoint=True`` generates the discrete-adjoint kernel instead (the
+    reverse-mode contractions of the callback; registers up to
+    ``ADJOINT_UNROLLED_MAX_VARS`` variables, the rolled scratch step above,
+    never LDS)."""
     if options.lu_precision not in ("fp32", "fp64"):
         raise ValueError(f"lu_precision must be 'fp32' or 'fp64', got {options.lu_precision!r}")
     traced = _codegen.trace(ode_fn, n_vars, n_params)
@@ -248,8 +253,15 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
     n_aug = n_vars if spec is None else spec.n_aug
     if adjoint and spec is not None:
         raise ValueError("the adjoint kernel differentiates the plain solve; drop the sensitivity spec")
-    place = rodas5p_placement(n_aug, options.lu_precision, block,
-                              "rolled" if adjoint else options.placement)
+    placement = options.placement
+    if adjoint:
+        # the adjoint reads the state from per-lane arrays: registers for a few
+        # variables (every stage loop unrolled), the rolled scratch step above
+        if placement is None:
+            placement = "registers" if n_aug <= ADJOINT_UNROLLED_MAX_VARS else "rolled"
+        elif placement == "lds":
+            placement = "rolled"
+    place = rodas5p_placement(n_aug, options.lu_precision, block, placement)
     defs = {
         "MDX_N": n_vars,
         "MDX_NP": n_params,

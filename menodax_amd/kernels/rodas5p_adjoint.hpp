@@ -2,9 +2,12 @@
 // trajectory per lane.
 //
 // Included after rodas5p.hpp in a source generated with MDX_ADJOINT=1 (the
-// forward step then writes no history) and MDX_ROLLED=1, whose sys namespace
-// also holds f_vjp / jac_vjp / dt_vjp (reverse-mode contractions of the
-// callback) and, for a sparse system, sparse_solve_t.
+// forward step then writes no history), whose sys namespace also holds f_vjp /
+// jac_vjp / dt_vjp (reverse-mode contractions of the callback) and, for a
+// sparse system, sparse_solve_t. With MDX_ROLLED=1 (larger systems) the stage
+// loops stay loops and the per-stage arrays live in scratch; with MDX_ROLLED=0
+// (a few variables) every stage loop is unrolled, so the stage vectors and
+// their adjoints are registers.
 //
 // Given dL/dhist for a loss of the saved history, each lane returns dL/dy0 and
 // dL/dparams of its trajectory, differentiating the solver's own arithmetic:
@@ -46,6 +49,12 @@ struct AdjointArgs {
   i64 gy_ts, gy_vs, gp_ts, gp_vs;
   int ckpt_every, max_ckpt;
 };
+
+#if MDX_ROLLED
+#define MDX_ADJ_STAGES _Pragma("unroll 1")
+#else
+#define MDX_ADJ_STAGES _Pragma("unroll")
+#endif
 
 constexpr int CK = N + 8;   // checkpoint words
 constexpr int RC = N + 4;   // step record words
@@ -108,7 +117,7 @@ __device__ void recompute(const double* __restrict__ y, double t, double h, PA& 
   for (int d = 0; d < N; ++d) lu[diag_slot(d)] = lu[diag_slot(d)] + (LUT)dtgamma_inv;
   factor(lu, ipiv);
   double u[N];
-#pragma unroll 1
+  MDX_ADJ_STAGES
   for (int S = 0; S < 8; ++S) {
     double ts;
     if (S == 0) {
@@ -198,7 +207,7 @@ __device__ void reverse_step(const AdjointArgs& g, i64 i, PA& p, const double* _
       for (int q = 0; q < 8; ++q) kbar[q][j] += (H1[q] * c1 + H2[q] * c2 + H3[q] * c3) * gj;
     }
   }
-#pragma unroll 1
+  MDX_ADJ_STAGES
   for (int S = 7; S >= 0; --S) {
     double rb[N], kd[N], u[N], ub[N];
     for (int j = 0; j < N; ++j) {

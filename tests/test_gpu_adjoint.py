@@ -90,6 +90,21 @@ def test_gradients_do_not_depend_on_the_checkpoint_interval():
         np.testing.assert_array_equal(got[1], ref[1])
 
 
+@pytest.mark.parametrize("lu", ["fp64", "fp32"])
+def test_adjoint_placements_give_the_same_bits(lu):
+    # the unrolled register kernel (chosen for 2 variables) against the rolled
+    # scratch kernel that larger systems use
+    fn, y0, mu, t, target, kw = _vdp_problem(1, 64, seed=4)
+    kw = dict(kw, lu_precision=lu)
+    hist = rodas5P.solve(fn, y0, t, mu, **kw)
+    _, g = _loss_and_cotangent(hist, target)
+    ref = rodas5P.adjoint_gradients(fn, y0, t, mu, g, ckpt_every=16, placement="rolled", **kw)
+    got = rodas5P.adjoint_gradients(fn, y0, t, mu, g, ckpt_every=16, **kw)
+    assert np.all(np.isfinite(ref[0])) and np.all(np.isfinite(ref[1]))
+    np.testing.assert_array_equal(got[0], ref[0])
+    np.testing.assert_array_equal(got[1], ref[1])
+
+
 def test_stiff_robertson_fp32_lu_adjoint_against_sensitivities():
     y0, p = S.robertson_lognormal_ensemble(64, seed=5)
     t = np.array([0.0, 1e-3, 1e-1, 10.0, 100.0])
