@@ -108,9 +108,10 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
 int64_t mdx_adjoint_workspace_bytes(int64_t n_traj, int32_t n_vars, int32_t ckpt_every,
                                     int32_t max_ckpt);
 
-/* The discrete adjoint of one ensemble solve: given gbar = dL/dhist
- * (N, n_save, n_vars) for the solve that mdx_solve_launch runs with the same
- * inputs, write dL/dy0 (N, n_vars) and dL/dparams (N, n_params). The kernel
+/* The discrete adjoint of one ensemble solve, Rodas5P or Tsit5 (the handle's
+ * kernel decides): given gbar = dL/dhist (N, n_save, n_vars) for the solve
+ * that mdx_solve_launch runs with the same inputs, write dL/dy0 (N, n_vars)
+ * and dL/dparams (N, n_params). The kernel
  * re-runs the solve storing a checkpoint before every ckpt_every-th accepted
  * step (at most max_ckpt per trajectory), then goes back through the accepted
  * steps segment by segment, recomputing each segment from its checkpoint; the

@@ -363,10 +363,17 @@ class Tsit5Options:
     placement: str | None = None
 
 
-def tsit5_source(ode_fn, n_vars: int, n_params: int, options: Tsit5Options):
+def tsit5_source(ode_fn, n_vars: int, n_params: int, options: Tsit5Options, adjoint=False):
+    """Generate the Tsit5 kernel source; returns (source, details).
+    ``adjoint=True`` generates the discrete-adjoint kernel instead (the
+    callback's reverse-mode contraction; registers up to
+    ``ADJOINT_UNROLLED_MAX_VARS`` variables, the rolled scratch step above,
+    never LDS)."""
     traced = _codegen.trace(ode_fn, n_vars, n_params)
     spec = options.sens
     n_error = n_vars
+    if adjoint and spec is not None:
+        raise ValueError("the adjoint kernel differentiates the plain solve; drop the sensitivity spec")
     if spec is not None:
         if (spec.n_vars, spec.n_params) != (n_vars, n_params):
             raise ValueError("the sensitivity spec was made for another system size")
@@ -375,7 +382,14 @@ def tsit5_source(ode_fn, n_vars: int, n_params: int, options: Tsit5Options):
     rhs_src, rhs_fc = _codegen.generate_rhs(traced)
     e1, e2, e3 = pid_exponents(TSIT_EXPONENT, options.pcoeff, options.icoeff, options.dcoeff)
     block = trajectories_per_block_or_default(options.trajectories_per_block)
-    place = tsit5_placement(n_vars, block, options.placement)
+    placement = options.placement
+    if adjoint:
+        # the adjoint reads the state from per-lane arrays
+        if placement is None:
+            placement = "registers" if n_vars <= ADJOINT_UNROLLED_MAX_VARS else "rolled"
+        elif placement == "lds":
+            placement = "rolled"
+    place = tsit5_placement(n_vars, block, placement)
     defs = {
         "MDX_N": n_vars,
         "MDX_NP": n_params,
@@ -391,8 +405,30 @@ def tsit5_source(ode_fn, n_vars: int, n_params: int, options: Tsit5Options):
         "MDX_LDS_U": place["MDX_LDS_U"],
         "MDX_LDS_Y": place["MDX_LDS_Y"],
     }
-    lines = _prelude(defs) + ["namespace sys {", rhs_src, "}  // namespace sys", '#include "tsit5.hpp"']
+    adj_src = ""
+    if adjoint:
+        defs["MDX_ADJOINT"] = 1
+        adj_src = _codegen.generate_adjoint(traced)
+    lines = _prelude(defs) + ["namespace sys {", rhs_src, adj_src, "}  // namespace sys",
+                              '#include "tsit5.hpp"']
+    if adjoint:
+        lines.append('#include "tsit5_adjoint.hpp"')
     return "\n".join(lines) + "\n", dict(rhs_flops=rhs_fc, block=block, placement=place)
+
+
+@functools.lru_cache(maxsize=None)
+def tsit5_adjoint_kernel(ode_fn, n_vars: int, n_params: int, options: Tsit5Options) -> Built:
+    """The discrete-adjoint kernel for the solve ``tsit5_kernel`` builds with
+    the same options; ``plain`` is its handle (``refill`` is 0)."""
+    source, d = tsit5_source(ode_fn, n_vars, n_params, options, adjoint=True)
+    code = _runtime.code_object(source, f"tsit5_adjoint_{getattr(ode_fn, '__name__', 'rhs')}.hip")
+    flags = _runtime.KERNEL_ADJOINT | (_runtime.KERNEL_UNIT_WEIGHTS if options.unit_weights else 0)
+    handle = _runtime.kernel_create(code, "mdx_tsit5_adjoint", d["block"], flags)
+    return Built(
+        method="tsit5_adjoint", plain=handle, refill=0, block=d["block"], n_vars=n_vars,
+        n_params=n_params, unit_weights=options.unit_weights, source=source,
+        rhs_flops=d["rhs_flops"], jac_flops=_codegen.FlopCount(), lu_precision="fp64",
+    )
 
 
 @functools.lru_cache(maxsize=None)

@@ -53,7 +53,7 @@ struct SolveArgs {
 };
 static_assert(sizeof(SolveArgs) == 10 * 8 + 3 * 8 + 8 + 11 * 8 + 8, "SolveArgs layout");
 
-// Mirrors mdx::r5::AdjointArgs in kernels/rodas5p_adjoint.hpp.
+// Mirrors mdx::AdjointArgs in kernels/mdx_common.hpp (both adjoint kernels).
 struct AdjointArgs {
   SolveArgs s;
   const double* gbar;

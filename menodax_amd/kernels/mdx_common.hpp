@@ -42,6 +42,20 @@ struct SolveArgs {
   int n_save;
 };
 
+// The discrete-adjoint launch (both methods): the forward solve's inputs, the
+// cotangent of its history, the gradient outputs and the checkpoint workspace.
+struct AdjointArgs {
+  SolveArgs s;             // the forward solve's inputs; hist/counters unused
+  const double* gbar;      // dL/dhist, strides g_ts, g_ss, g_vs
+  double* grad_y0;         // (traj, var), strides gy_ts, gy_vs
+  double* grad_p;          // (traj, param), strides gp_ts, gp_vs
+  int* status;             // (traj,): 0 ok, 1 checkpoint space exhausted
+  double* ws;              // workspace
+  i64 g_ts, g_ss, g_vs;
+  i64 gy_ts, gy_vs, gp_ts, gp_vs;
+  int ckpt_every, max_ckpt;
+};
+
 // A trajectory's save-hook accumulator row in global memory.
 struct HookRow {
   double* base;

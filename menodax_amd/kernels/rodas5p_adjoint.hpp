@@ -38,17 +38,6 @@
 namespace mdx {
 namespace r5 {
 
-struct AdjointArgs {
-  SolveArgs s;             // the forward solve's inputs; hist/counters unused
-  const double* gbar;      // dL/dhist, strides g_ts, g_ss, g_vs
-  double* grad_y0;         // (traj, var), strides gy_ts, gy_vs
-  double* grad_p;          // (traj, param), strides gp_ts, gp_vs
-  int* status;             // (traj,): 0 ok, 1 checkpoint space exhausted
-  double* ws;              // workspace
-  i64 g_ts, g_ss, g_vs;
-  i64 gy_ts, gy_vs, gp_ts, gp_vs;
-  int ckpt_every, max_ckpt;
-};
 
 #if MDX_ROLLED
 #define MDX_ADJ_STAGES _Pragma("unroll 1")
@@ -255,7 +244,7 @@ __device__ void reverse_step(const AdjointArgs& g, i64 i, PA& p, const double* _
 }  // namespace mdx
 
 extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_rodas5p_adjoint(
-    mdx::r5::AdjointArgs g) {
+    mdx::AdjointArgs g) {
   using namespace mdx::r5;
   const mdx::SolveArgs& a = g.s;
   const mdx::i64 i = (mdx::i64)blockIdx.x * MDX_BLOCK + threadIdx.x;

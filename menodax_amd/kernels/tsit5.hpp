@@ -19,6 +19,10 @@
 //                  carried between steps), the stage argument and the state
 //                  in LDS [element][lane] as far as MDX_LDS_K/U/Y say, else in
 //                  scratch. Same arithmetic, same bits.
+//
+// MDX_ADJOINT 1 builds the source of the discrete-adjoint kernel
+// (tsit5_adjoint.hpp): the step writes no history and the solve kernels are
+// left out.
 #pragma once
 #include "mdx_common.hpp"
 
@@ -64,8 +68,12 @@ constexpr double E1 = 0.0017800110522257773, E2 = 0.0008164344596567463,
                  E3 = -0.007880878010261994, E4 = 0.1447110071732629, E5 = -0.5823571654525552,
                  E6 = 0.45808210592918686, E7 = -1.0 / 66.0;
 
-#if MDX_ROLLED
-// Stage rows 2..7 (the last is the solution weights B) and abscissae.
+#ifndef MDX_ADJOINT
+#define MDX_ADJOINT 0
+#endif
+
+// Stage rows 2..7 (the last is the solution weights B) and abscissae, for the
+// rolled step and the adjoint.
 __device__ constexpr double AT[7][6] = {
     {0, 0, 0, 0, 0, 0},
     {A21, 0, 0, 0, 0, 0},
@@ -77,6 +85,22 @@ __device__ constexpr double AT[7][6] = {
 };
 __device__ constexpr double CS[7] = {0.0, C2, C3, C4, C5, C6, C7};
 
+// Weights b_1..b_7 of the continuous extension at theta (tsit5.py:309-361).
+__device__ __forceinline__ void dense_weights(double th, double* __restrict__ b) {
+  b[0] = -1.0530884977290216 * th * (th - 1.3299890189751412) *
+         (th * th - 1.4364028541716351 * th + 0.7139816917074209);
+  b[1] = 0.1017 * th * th * (th * th - 2.1966568338249754 * th + 1.2949852507374631);
+  b[2] = 2.490627285651252793 * th * th *
+         (th * th - 2.38535645472061657 * th + 1.57803468208092486);
+  b[3] = -16.54810288924490272 * (th - 1.21712927295533244) * (th - 0.61620406037800089) * th *
+         th;
+  b[4] = 47.37952196281928122 * (th - 1.203071208372362603) * (th - 0.658047292653547382) * th *
+         th;
+  b[5] = -34.87065786149660974 * (th - 1.2) * (th - 0.666666666666666667) * th * th;
+  b[6] = 2.5 * (th - 1.0) * (th - 0.6) * th * th;
+}
+
+#if MDX_ROLLED
 __device__ __forceinline__ unsigned opaque_lane() {
   unsigned v = threadIdx.x;
   asm volatile("; lane" : "+v"(v));
@@ -132,14 +156,18 @@ __device__ __forceinline__ void init(State& s, const SolveArgs& a, i64 i) {
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     YJ(s, j) = a.y0[i * a.y0_ts + (i64)j * a.y0_vs];
+#if !MDX_ADJOINT
     a.hist[i * a.h_ts + (i64)j * a.h_vs] = YJ(s, j);
+#endif
     KQ(s, 6, j) = 0.0;
   }
 #else
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     s.y[j] = a.y0[i * a.y0_ts + (i64)j * a.y0_vs];
+#if !MDX_ADJOINT
     a.hist[i * a.h_ts + (i64)j * a.h_vs] = s.y[j];
+#endif
     s.k7[j] = 0.0;
   }
 #endif
@@ -254,30 +282,24 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
       const double at = absT(ts);
       if (!(ts <= t_new + 1e-12 * (1.0 > at ? 1.0 : at))) break;
       const double th = (ts - t) / dt_use;
-      const double b1 = -1.0530884977290216 * th * (th - 1.3299890189751412) *
-                        (th * th - 1.4364028541716351 * th + 0.7139816917074209);
-      const double b2 =
-          0.1017 * th * th * (th * th - 2.1966568338249754 * th + 1.2949852507374631);
-      const double b3 = 2.490627285651252793 * th * th *
-                        (th * th - 2.38535645472061657 * th + 1.57803468208092486);
-      const double b4 = -16.54810288924490272 * (th - 1.21712927295533244) *
-                        (th - 0.61620406037800089) * th * th;
-      const double b5 = 47.37952196281928122 * (th - 1.203071208372362603) *
-                        (th - 0.658047292653547382) * th * th;
-      const double b6 = -34.87065786149660974 * (th - 1.2) * (th - 0.666666666666666667) * th * th;
-      const double b7 = 2.5 * (th - 1.0) * (th - 0.6) * th * th;
+#if !MDX_ADJOINT
+      double b[7];
+      dense_weights(th, b);
       double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        double q = b1 * KQ(s, 0, j);
-        q = fmaT(b2, KQ(s, 1, j), q);
-        q = fmaT(b3, KQ(s, 2, j), q);
-        q = fmaT(b4, KQ(s, 3, j), q);
-        q = fmaT(b5, KQ(s, 4, j), q);
-        q = fmaT(b6, KQ(s, 5, j), q);
-        q = fmaT(b7, KQ(s, 6, j), q);
+        double q = b[0] * KQ(s, 0, j);
+        q = fmaT(b[1], KQ(s, 1, j), q);
+        q = fmaT(b[2], KQ(s, 2, j), q);
+        q = fmaT(b[3], KQ(s, 3, j), q);
+        q = fmaT(b[4], KQ(s, 4, j), q);
+        q = fmaT(b[5], KQ(s, 5, j), q);
+        q = fmaT(b[6], KQ(s, 6, j), q);
         row[(i64)j * a.h_vs] = fmaT(dt_use, q, YJ(s, j));
       }
+#else
+      (void)th;
+#endif
       s.save_idx++;
     }
 #pragma unroll
@@ -382,31 +404,24 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
       const double at = absT(ts);
       if (!(ts <= t_new + 1e-12 * (1.0 > at ? 1.0 : at))) break;
       const double th = (ts - t) / dt_use;
-      // tsit5.py:309-361
-      const double b1 = -1.0530884977290216 * th * (th - 1.3299890189751412) *
-                        (th * th - 1.4364028541716351 * th + 0.7139816917074209);
-      const double b2 =
-          0.1017 * th * th * (th * th - 2.1966568338249754 * th + 1.2949852507374631);
-      const double b3 = 2.490627285651252793 * th * th *
-                        (th * th - 2.38535645472061657 * th + 1.57803468208092486);
-      const double b4 = -16.54810288924490272 * (th - 1.21712927295533244) *
-                        (th - 0.61620406037800089) * th * th;
-      const double b5 = 47.37952196281928122 * (th - 1.203071208372362603) *
-                        (th - 0.658047292653547382) * th * th;
-      const double b6 = -34.87065786149660974 * (th - 1.2) * (th - 0.666666666666666667) * th * th;
-      const double b7 = 2.5 * (th - 1.0) * (th - 0.6) * th * th;
+#if !MDX_ADJOINT
+      double b[7];
+      dense_weights(th, b);
       double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        double q = b1 * k1[j];
-        q = fmaT(b2, k2[j], q);
-        q = fmaT(b3, k3[j], q);
-        q = fmaT(b4, k4[j], q);
-        q = fmaT(b5, k5[j], q);
-        q = fmaT(b6, k6[j], q);
-        q = fmaT(b7, s.k7[j], q);
+        double q = b[0] * k1[j];
+        q = fmaT(b[1], k2[j], q);
+        q = fmaT(b[2], k3[j], q);
+        q = fmaT(b[3], k4[j], q);
+        q = fmaT(b[4], k5[j], q);
+        q = fmaT(b[5], k6[j], q);
+        q = fmaT(b[6], s.k7[j], q);
         row[(i64)j * a.h_vs] = fmaT(dt_use, q, s.y[j]);
       }
+#else
+      (void)th;
+#endif
       s.save_idx++;
     }
 #pragma unroll
@@ -443,6 +458,7 @@ __device__ __forceinline__ void finish(State& s, const SolveArgs& a, i64 i) {
 }  // namespace t5
 }  // namespace mdx
 
+#if !MDX_ADJOINT
 extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_tsit5(mdx::SolveArgs a) {
   using namespace mdx::t5;
   const mdx::i64 i = (mdx::i64)blockIdx.x * MDX_BLOCK + threadIdx.x;
@@ -483,3 +499,4 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_tsit5_refill(mdx::So
     }
   }
 }
+#endif  // !MDX_ADJOINT

@@ -8,7 +8,10 @@ per-trajectory vectors in shared memory and "local" in thread-local arrays
 every system the shared backend accepted, so both names select the same kernel
 there -- and above that a rolled stage loop with the stage vectors, stage
 argument and state in LDS as far as they fit. Placements are bit-identical.
-Differentiation and ``torch.func.vmap`` work as for ``rodas5P.solve``.
+Differentiation and ``torch.func.vmap`` work as for ``rodas5P.solve``:
+gradients come from the forward-sensitivity joint solve by default, or, with
+``adjoint=True``, from the discrete-adjoint kernel (checkpointed every
+``ckpt_every`` accepted steps), which ``adjoint_gradients`` also runs directly.
 """
 
 from __future__ import annotations
@@ -56,6 +59,8 @@ def solve(
     trajectories_per_block=None,
     refill=None,
     stream=None,
+    adjoint=False,
+    ckpt_every=64,
 ):
     """Integrate an ensemble with Tsit5; see the module docstring."""
     if _autodiff.under_vmap(y0, t_span, params):
@@ -91,6 +96,18 @@ def solve(
     launch_kw = dict(dt0=_common.initial_step(first_step), rtol=rtol, atol=atol,
                      max_steps=max_steps, refill=refill, stream=stream)
     wrt_y0, wrt_params = _autodiff.wanted(y0, params)
+    if (wrt_y0 or wrt_params) and adjoint:
+        _check_backend(n_vars, backend)
+        job = _autodiff.AdjointJob(
+            built=_kernels.tsit5_kernel(ode_fn, n_vars, n_params, options),
+            adjoint=_kernels.tsit5_adjoint_kernel(ode_fn, n_vars, n_params, options),
+            times=times, n=n, n_vars=n_vars, n_params=n_params,
+            weights=_common.error_weights_soa(error_weights, n, n_vars, dev),
+            launch_kw=launch_kw, ckpt_every=int(ckpt_every))
+        hist, counters = _autodiff.adjoint_solve(job, y0_t, p_t)
+        if return_stats:
+            return hist, _common.solver_stats(*counters)
+        return hist
     if wrt_y0 or wrt_params:
         job = _joint_job(ode_fn, options, backend, times, n, n_vars, n_params, error_weights,
                          dev, launch_kw)
@@ -121,6 +138,48 @@ def _joint_job(ode_fn, options, backend, times, n, n_vars, n_params, error_weigh
 
     return _autodiff.JointJob(build=build, times=times, n=n, n_vars=n_vars, n_params=n_params,
                               weights=w, launch_kw=launch_kw)
+
+
+def adjoint_gradients(ode_fn, y0, t_span, params, hist_cotangent, *, ckpt_every=64,
+                      rtol=1e-8, atol=1e-10, first_step=None, max_steps=100000,
+                      error_weights=None, pcoeff=0.0, icoeff=1.0, dcoeff=0.0, backend="auto",
+                      trajectories_per_block=None, stream=None, placement=None):
+    """Gradients of a loss of the Tsit5 solve's history by the discrete
+    adjoint; arguments and return as ``rodas5P.adjoint_gradients``:
+    ``hist_cotangent`` is dL/dhist ``(N, n_save, n_vars)`` for the solve with
+    the same options, the result ``(dL/dy0 (N, n_vars), dL/dparams (N,
+    n_params))``. Accepted steps are differentiated with their sizes held
+    fixed; saves the solve never reached (NaN in its history) carry no
+    derivative. ``placement`` chooses where the kernels keep their per-lane
+    vectors (by size when None); every placement gives the same bits."""
+    dev = _common.device()
+    y0_t, p_t, n, n_vars = _common.normalize_y0_params(y0, params, dev)
+    n_params = p_t.shape[1]
+    _check_backend(n_vars, backend)
+    options = Tsit5Options(
+        pcoeff=float(pcoeff), icoeff=float(icoeff), dcoeff=float(dcoeff),
+        trajectories_per_block=_kernels.trajectories_per_block_or_default(trajectories_per_block),
+        unit_weights=error_weights is None, placement=placement,
+    )
+    times = _common.as_f64_device(t_span, dev).contiguous()
+    launch_kw = dict(dt0=_common.initial_step(first_step), rtol=rtol, atol=atol,
+                     max_steps=max_steps, refill=False, stream=stream)
+    weights = _common.error_weights_soa(error_weights, n, n_vars, dev)
+    job = _autodiff.AdjointJob(
+        built=_kernels.tsit5_kernel(ode_fn, n_vars, n_params, options),
+        adjoint=_kernels.tsit5_adjoint_kernel(ode_fn, n_vars, n_params, options),
+        times=times, n=n, n_vars=n_vars, n_params=n_params, weights=weights,
+        launch_kw=launch_kw, ckpt_every=int(ckpt_every))
+    y0_d, p_d = y0_t.detach(), p_t.detach()
+    out = launch(job.built, y0_d, times, p_d, n, weights=weights, **launch_kw)
+    g = _common.as_f64_device(hist_cotangent, dev)
+    if tuple(g.shape) != (n, times.numel(), n_vars):
+        raise ValueError(f"hist_cotangent must be (N, n_save, n_vars) = "
+                         f"{(n, times.numel(), n_vars)}; got {tuple(g.shape)}")
+    gy, gp = _autodiff.adjoint_gradients(job, y0_d, p_d, g, out.accepted)
+    if _common.wants_numpy(y0, params, t_span, hist_cotangent):
+        return gy.cpu().numpy(), gp.cpu().numpy()
+    return gy, gp
 
 
 def solve_sensitivities(ode_fn, y0, t_span, params, *, wrt_y0=True, wrt_params=True,
