@@ -440,6 +440,16 @@ def main():
         )
         sys.exit(2)
 
+    scipy_pool = None
+    if not (args.no_cpu_baseline or args.only_uneven_stiff or args.calibrate_copy
+            or args.lu != "both"):
+        # worker processes for the SciPy CPU baseline, spawned while this
+        # process has not yet initialised the GPU
+        from oracle import oracle as O
+        from oracle import scipy_baseline as SB
+
+        scipy_pool = SB.Pool(O.max_threads())
+
     import torch
 
     from menodax_amd import _kernels, _runtime, roofline, rodas5P, systems as S
@@ -623,6 +633,29 @@ def main():
                       + (f" (all of it, solved {reps} times)" if reps > 1 else "")
                       + f", FP32-LU, C oracle with OpenMP, {dt:.3f} s per solve",
         }
+        # the reference's own kind of CPU baseline: SciPy solve_ivp per
+        # trajectory over a pool of single-threaded worker processes (started
+        # before this process touched the GPU)
+        pool = scipy_pool
+        try:
+            n_sp, dt_sp, _ = SB.timed_sample(pool, "robertson", y0_np, times_np, p_np,
+                                             args.cpu_seconds / 3, rtol=RTOL, atol=ATOL)
+            line["cpu_baseline_scipy"] = {
+                "value": n_sp / dt_sp, "unit": "trajectories/s", "cores": threads,
+                "kind": "port", "method": "LSODA",
+                "sample": f"first {n_sp} trajectories of the same ensemble, scipy.integrate."
+                          f"solve_ivp (LSODA) per trajectory over {threads} single-threaded "
+                          f"worker processes, NumPy right-hand side, {dt_sp:.2f} s",
+            }
+            y0l, rhol = S.lorenz_rho_ensemble(1024)
+            n_l, dt_l, _ = SB.timed_sample(pool, "lorenz", y0l, np.linspace(0.0, 1.0, 100), rhol,
+                                           args.cpu_seconds / 3, method="RK45", rtol=RTOL,
+                                           atol=ATOL)
+            lorenz_scipy = {"value": n_l / dt_l, "unit": "trajectories/s", "cores": threads,
+                            "kind": "port", "method": "RK45",
+                            "sample": f"{n_l} trajectories, {dt_l:.2f} s"}
+        finally:
+            pool.close()
 
     if not args.skip_reference_config:
         # the reference's own published inputs (stiff_robertson_scaling, N=100000)
@@ -660,6 +693,8 @@ def main():
     if not args.skip_adjoint:
         line["adjoint"] = adjoint_block(torch, log)
     line["lorenz_tsit5"] = lorenz_block(torch, dev)
+    if not args.no_cpu_baseline:
+        line["lorenz_tsit5"]["cpu_baseline_scipy"] = lorenz_scipy
     if args.vdp_chain:
         line["vdp_chain"] = vdp_chain_block(torch, dev, log)
 
