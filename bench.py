@@ -183,6 +183,25 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64, variants=UNEVEN_VAR
             log_fn(f"uneven stiff {key}: {ms:.1f} ms")
     if not {"fp32_refill", "fp64_plain"} <= set(res):
         return {"variants": res}
+    # the autotuner over placement x workgroup size on the whole ensemble, in
+    # the launch shape solve picks for it, every candidate checked
+    # bit-identical, the fastest kept
+    from menodax_amd import autotune
+
+    k = n
+    best, cands = autotune.tune(fn, y0_t, t_t, p_t, reps=1, rtol=RTOL, atol=ATOL, shapes="solve",
+                                max_steps=100000, lu_precision="fp32", sparsity=mask)
+    default_block = _kernels.trajectories_per_block_or_default(None)
+    default = next(c for c in cands if c.placement == res["fp32_refill"]["placement"]
+                   and c.trajectories_per_block == default_block)
+    tuned = {"sample": k, "chosen": {"placement": best.placement,
+                                     "trajectories_per_block": best.trajectories_per_block,
+                                     "refill": best.refill, "sample_ms": best.seconds * 1e3},
+             "default_sample_ms": default.seconds * 1e3,
+             "candidates": [{"placement": c.placement, "block": c.trajectories_per_block,
+                             "refill": c.refill, "ms": c.seconds * 1e3} for c in cands]}
+    log_fn(f"autotune: {best.placement} block {best.trajectories_per_block} refill {best.refill} "
+           f"{best.seconds*1e3:.1f} ms on {k} (default {default.seconds*1e3:.1f} ms)")
     steps = att
     edges = np.logspace(np.log10(max(1, steps.min())), np.log10(steps.max() + 1), 9)
     counts, _ = np.histogram(steps, bins=edges)
@@ -217,6 +236,7 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64, variants=UNEVEN_VAR
                             "histogram_edges": [float(e) for e in edges],
                             "histogram_counts": [int(c) for c in counts]},
         "endpoint_error": {"sample": sample, **err},
+        "autotune": tuned,
     }
 
 
@@ -239,11 +259,23 @@ def lorenz_block(torch, dev, n=1024):
 
     o, wall, ev_ms = time_solves(run, 20, 3, torch)
     att = (o.accepted + o.rejected).float()
+    # endpoint error on a sample: the FP64 oracle at the same tolerance (the
+    # same arithmetic) and a tight one
+    from oracle import oracle as O
+
+    k = 64
+    tn = np.linspace(0.0, 1.0, 100)
+    same, _ = O.tsit5(O.lorenz(), y0[:k], tn, p[:k], rtol=RTOL, atol=ATOL)
+    tight, _ = O.tsit5(O.lorenz(), y0[:k], tn, p[:k], rtol=1e-12, atol=1e-14)
+    yg = o.ensemble_view()[:k, -1].cpu().numpy()
     return {"workload": "Tsit5, Lorenz, 1,024 trajectories, rho ~ U[0,28], y0=(1,0,0), t in [0,1], "
                         "100 saves, rtol 1e-6 atol 1e-8, fp64",
             "ms": wall / 20 * 1e3, "kernel_ms": ev_ms,
             "trajectories_per_s": n / (wall / 20),
-            "attempted_mean": float(att.mean().item()), "attempted_max": int(att.max().item())}
+            "attempted_mean": float(att.mean().item()), "attempted_max": int(att.max().item()),
+            "endpoint_error": {"sample": k,
+                               "max_abs_vs_fp64_oracle_same_tol": float(np.max(np.abs(yg - same[:, -1]))),
+                               "max_abs_vs_tight_oracle": float(np.max(np.abs(yg - tight[:, -1])))}}
 
 
 def vdp_chain_block(torch, dev, log_fn, n=16384):
@@ -279,6 +311,28 @@ def vdp_chain_block(torch, dev, log_fn, n=16384):
                     "attempted_mean": float(att.mean()), "attempted_max": int(att.max()),
                     "hit_max_steps": int((o.loop.cpu().numpy() >= 100000).sum())}
         log_fn(f"vdp chain {key}: {sec*1e3:.1f} ms")
+    # endpoint error on the eight cheapest trajectories: the oracle with the
+    # same sparse FP32 LU and tolerance. Against a tight solve the endpoint
+    # says little here: over [0, 2000] a trajectory runs through up to ~125
+    # relaxation periods and a phase difference of a fraction of one moves the
+    # endpoint by O(1) (the oracle at rtol 1e-6 and at 1e-10 differ by 13 at
+    # t = 2000 for mu = 10)
+    from menodax_amd import _sparse_direct as SD
+    from oracle import oracle as O
+
+    att = (o.accepted + o.rejected).cpu().numpy()
+    idx = np.argsort(att)[:8]
+    system = O.vdp_lattice(32, mu=1.0, d=0.1, wrap=False)
+    same, st = O.rodas5p(system, y0[idx], np.linspace(0.0, 2000.0, 11), mu[idx],
+                         layout=SD.oracle_tables(mask, "amd"), rtol=RTOL, atol=ATOL,
+                         lu_precision="fp32")
+    hist = o.ensemble_view()[idx].cpu().numpy()
+    res["endpoint_error"] = {
+        "sample": "the 8 trajectories with the fewest steps",
+        "max_abs_vs_oracle_same_tol_fp32_lu": float(np.max(np.abs(hist[:, -1] - same[:, -1]))),
+        "history_bit_identical_to_oracle": bool(np.array_equal(hist, same)),
+        "counters_equal_to_oracle": bool(np.array_equal(att[idx], st["accepted_steps"]
+                                                         + st["rejected_steps"]))}
     return res
 
 

@@ -88,7 +88,10 @@ int64_t mdx_workspace_bytes(int64_t handle, int64_t n_traj);
  * passes them. `layout` 1: structure of arrays -- y0/weights (n_vars, N),
  * params (n_params, N), hist (n_save, n_vars, N) -- the coalesced form.
  * `grid` 0 sizes a one-lane-per-trajectory launch from N, or a persistent
- * kernel's resident grid. `weights` may be NULL only for a kernel registered
+ * kernel's resident grid; a positive `grid` sets the workgroup count (any
+ * count for a persistent kernel, whose lanes claim trajectories from the
+ * queue; at least ceil(N / block) for a one-lane-per-trajectory kernel, which
+ * would otherwise leave trajectories unsolved, so a smaller one is an error). `weights` may be NULL only for a kernel registered
  * with MDX_KERNEL_UNIT_WEIGHTS. `workspace` must hold mdx_workspace_bytes; it
  * is reset on the stream before the solve. A kernel registered with
  * MDX_KERNEL_HOOK zeroes and accumulates hook_out (N, hook_size), laid out like

@@ -8,6 +8,19 @@ footprint and the spread of step counts, and is measured rather than guessed:
 ``tune`` times each candidate on a sample of the ensemble, checks that they all
 agree bit for bit, and returns the winner's options, cached per (callback,
 sizes, options) for the process.
+
+A sample is a different regime from the whole ensemble for the launch shape: a
+persistent grid sized for the GPU would give every sampled trajectory a lane of
+its own. So when timing a sample of k out of N trajectories, the launch shape
+is the one ``solve`` picks for N (persistent refill once N exceeds the lanes
+the persistent grid keeps resident), and a persistent launch runs on its
+resident grid scaled by k/N, each lane working through as many trajectories as
+it would in the whole solve. That keeps the work per lane, not the number of
+workgroups sharing a compute unit: a sample times a placement with several
+resident workgroups per CU without their contention, so it ranks such
+placements optimistically. With no sampling both shapes are timed, or with
+``shapes="solve"`` only the one ``solve`` picks -- exact, at the cost of a
+whole-ensemble launch per candidate.
 """
 
 from __future__ import annotations
@@ -18,7 +31,8 @@ from dataclasses import dataclass, replace
 from . import _common, _kernels
 from ._kernels import PLACEMENTS, KernelOptions, rodas5p_placement
 from ._sparsity import normalize_sparsity
-from .rodas5P import launch
+from . import _runtime
+from .rodas5P import _use_refill, launch
 
 _CACHE: dict = {}
 
@@ -48,15 +62,17 @@ def candidates(n_vars: int, lu_precision: str, blocks=(64, 128, 256)):
 
 def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-10,
          first_step=None, max_steps=100000, lu_precision="fp32", sparsity=None, ordering="amd",
-         pcoeff=0.0, icoeff=1.0, dcoeff=0.0, tf_index=None, blocks=(64, 128, 256)):
-    """Time every candidate on ``sample`` trajectories (all of them by default)
-    and return ``(best Choice, [Choice for every candidate])``. Raises if two
-    candidates disagree in any bit."""
+         pcoeff=0.0, icoeff=1.0, dcoeff=0.0, tf_index=None, blocks=(64, 128, 256), shapes=None):
+    """Time every candidate on ``sample`` trajectories (all of them by default;
+    see the module docstring for the launch shape of a sample) and return
+    ``(best Choice, [Choice for every candidate])``. Raises if two candidates
+    disagree in any bit."""
     import numpy as np
     import torch
 
     dev = _common.device()
     y0_t, p_t, n, n_vars = _common.normalize_y0_params(y0, params, dev)
+    n_full = n
     if sample is not None and sample < n:
         y0_t, p_t, n = y0_t[:sample], p_t[:sample], int(sample)
     n_params = p_t.shape[1]
@@ -65,7 +81,9 @@ def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-
     base = KernelOptions(pcoeff=float(pcoeff), icoeff=float(icoeff), dcoeff=float(dcoeff),
                          lu_precision=lu_precision, sparsity=sparsity, ordering=ordering,
                          tf_index=-1 if tf_index is None else int(tf_index))
-    key = (ode_fn, n_vars, n_params, base, n, tuple(blocks))
+    if shapes not in (None, "solve"):
+        raise ValueError(f"shapes must be None or 'solve', got {shapes!r}")
+    key = (ode_fn, n_vars, n_params, base, n, n_full, tuple(blocks), shapes)
     if key in _CACHE:
         return _CACHE[key]
     times = _common.as_f64_device(t_span, dev).contiguous()
@@ -75,8 +93,17 @@ def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-
         built = _kernels.rodas5p_kernel(
             ode_fn, n_vars, n_params,
             replace(base, placement=placement, trajectories_per_block=block))
-        for refill in (False, True):
-            out = launch(built, y0_t, times, p_t, n, refill=refill, **kw)
+        launches = [(False, 0), (True, 0)]
+        if n < n_full or shapes == "solve":
+            if not _use_refill(built, n_full):
+                launches = [(False, 0)]
+            elif n < n_full:
+                launches = [(True, max(1, round(_runtime.persistent_grid(built.refill)
+                                                * n / n_full)))]
+            else:
+                launches = [(True, 0)]
+        for refill, grid in launches:
+            out = launch(built, y0_t, times, p_t, n, refill=refill, grid=grid, **kw)
             torch.cuda.synchronize()
             hist = out.hist.cpu().numpy()
             if reference is None:
@@ -87,7 +114,8 @@ def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-
             ws = out._keep[-1]
             t0 = time.perf_counter()
             for _ in range(reps):
-                launch(built, y0_t, times, p_t, n, refill=refill, outputs=out, workspace=ws, **kw)
+                launch(built, y0_t, times, p_t, n, refill=refill, grid=grid, outputs=out,
+                       workspace=ws, **kw)
             torch.cuda.synchronize()
             results.append(Choice(placement, block, refill, (time.perf_counter() - t0) / reps))
     best = min(results, key=lambda c: c.seconds)

@@ -287,6 +287,9 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
     return fail(1, "mdx_solve_launch: this kernel runs a save hook; hook_out and hook_size are required");
   if (max_steps < 0) return fail(1, "mdx_solve_launch: max_steps must be non-negative");
   if (k->adjoint) return fail(1, "mdx_solve_launch: an adjoint kernel is launched by mdx_adjoint_launch");
+  if (grid > 0 && !k->persistent && (int64_t)grid * k->block < n_traj)
+    return fail(1, "mdx_solve_launch: a one-lane-per-trajectory launch needs grid >= "
+                   "ceil(n_traj / block)");
   if (k->persistent && (!workspace || workspace_bytes < 8))
     return fail(1, "mdx_solve_launch: a persistent kernel needs a workspace of at least 8 bytes");
   if (n_traj == 0) return 0;

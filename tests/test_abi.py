@@ -67,6 +67,7 @@ def _launch(handle, **over):
         (dict(weights=0), "reads error weights"),
         (dict(max_steps=-3), "max_steps"),
         (dict(hook_size=2), "needs hook_out"),
+        (dict(n_traj=200, grid=3), "grid >= ceil"),
     ],
 )
 def test_malformed_launches_are_rejected_before_the_gpu(handles, override, message):

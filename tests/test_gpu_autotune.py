@@ -19,3 +19,21 @@ def test_tuned_options_reproduce_the_default_solve():
     opts = autotune.tuned_options(fn, y0, t, lam, blocks=(64, 128), **kw)
     tuned = rodas5P.solve(fn, y0, t, lam, **opts, **kw)
     np.testing.assert_array_equal(tuned, rodas5P.solve(fn, y0, t, lam, **kw))
+
+
+def test_a_sample_is_timed_in_the_launch_shape_of_the_whole_ensemble():
+    from menodax_amd import _kernels
+    from menodax_amd._sparsity import normalize_sparsity
+
+    fn, mask, _ = S.arrow_system(24)
+    y0, lam = S.arrow_ensemble(4096, n=24, lam_range=(1e-1, 1e3))
+    t = np.linspace(0.0, 2.0, 5)
+    kw = dict(rtol=1e-6, atol=1e-8, lu_precision="fp32", sparsity=mask)
+    best, table = autotune.tune(fn, y0, t, lam, sample=1024, blocks=(64, 128), **kw)
+    pairs = [(c.placement, c.trajectories_per_block) for c in table]
+    assert len(pairs) == len(set(pairs)), "one launch shape per candidate when sampling"
+    for c in table:
+        built = _kernels.rodas5p_kernel(fn, 24, lam.shape[1], _kernels.KernelOptions(
+            lu_precision="fp32", sparsity=normalize_sparsity(mask, 24), placement=c.placement,
+            trajectories_per_block=c.trajectories_per_block))
+        assert c.refill == rodas5P._use_refill(built, 4096)
