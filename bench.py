@@ -458,6 +458,84 @@ def tsit5_adjoint_line(torch, log_fn, n, ckpt_every):
             "adjoint_vs_central_difference_max_rel": rel_fd}
 
 
+# The reference's published RTX 4070 SUPER times (ms) for its own benchmark
+# scripts, and where they are recorded (paths under the reference's benchmarks/).
+REF_PUBLISHED_MS = {
+    "lorenz_tsit5_n100000": (93.62, "nonstiff_lorenz_scaling/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:11"),
+    "lorenz_tsit5_n1000": (3.26, "nonstiff_lorenz_scaling/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:7"),
+    "vdp_ring_n32_sparse_fp32": (1.60, "stiff_vdp_dimensionality/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:9"),
+    "vdp_ring_n32_dense_fp32": (43.44, "stiff_vdp_dimensionality/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:20"),
+    "vdp_ring_n32_dense_fp64": (52.98, "stiff_vdp_dimensionality/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:31"),
+    "vdp_ring_n64_sparse_fp32": (5.94, "stiff_vdp_dimensionality/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:10"),
+    "vdp_ring_n64_dense_fp32": (380.06, "stiff_vdp_dimensionality/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:21"),
+    "vdp_ring_n64_dense_fp64": (334.02, "stiff_vdp_dimensionality/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:32"),
+    "vdp_ring_n64_value_and_grad_sparse": (14.69, "stiff_vdp_gradient/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:10"),
+    "vdp_ring_n64_value_and_grad_dense": (515.08, "stiff_vdp_gradient/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:21"),
+}
+
+
+def reference_benchmarks(torch, log_fn, reps=5):
+    """The reference's own benchmark points with published times, run through
+    the public API on device-resident inputs (one warm-up, then the mean of
+    ``reps`` synchronised calls, as its time_blocked does): Lorenz Tsit5 on
+    identical trajectories (y0 = (1000, -500, 500), rho = 0.5, t in [0, 5],
+    first_step 1e-4); the stiff Van der Pol ring (mu = 100, d = 10, 1,000
+    identical trajectories, saves at 0, 0.25, ..., 1, first_step 1e-4) with
+    the sparse FP32, dense FP32 and dense FP64 LU; and the value and gradient
+    of the summed final state with respect to every trajectory's parameter by
+    forward sensitivities (the reference's jax.value_and_grad), sparse and
+    dense. rtol 1e-6, atol 1e-8 throughout."""
+    from menodax_amd import rodas5P, systems as S, tsit5
+
+    dev = torch.device("cuda", 0)
+
+    def clock(f):
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    out = {}
+
+    def record(key, ms, **extra):
+        pub, src = REF_PUBLISHED_MS[key]
+        out[key] = {"ms": ms, "published_rtx4070_super_ms": pub, "speedup": pub / ms,
+                    "source": src, **extra}
+        log_fn(f"reference benchmark {key}: {ms:.3f} ms (published {pub} ms)")
+
+    kw = dict(first_step=1e-4, rtol=1e-6, atol=1e-8)
+    t_l = torch.tensor(S.LORENZ_TIMES, device=dev)
+    for n in (100000, 1000):
+        y0 = torch.tensor(np.tile(S.LORENZ_Y0, (n, 1)), device=dev)
+        p = torch.tensor(np.tile(S.LORENZ_PARAMS, (n, 1)), device=dev)
+        record(f"lorenz_tsit5_n{n}", clock(lambda: tsit5.solve(S.lorenz, y0, t_l, p, **kw)))
+    t_v = torch.tensor(S.VDP_TIMES, device=dev)
+    for n_osc in (16, 32):
+        fn, _ = S.vdp_lattice(n_osc)
+        mask = S.vdp_sparsity(n_osc)
+        y0n, pn = S.vdp_scenario(n_osc, 1000, divergence=0.0)
+        y0, p = torch.tensor(y0n, device=dev), torch.tensor(pn, device=dev)
+        nv = 2 * n_osc
+        cases = (("sparse_fp32", dict(lu_precision="fp32", sparsity=mask)),
+                 ("dense_fp32", dict(lu_precision="fp32")),
+                 ("dense_fp64", dict(lu_precision="fp64")))
+        for name, extra in cases:
+            record(f"vdp_ring_n{nv}_{name}",
+                   clock(lambda: rodas5P.solve(fn, y0, t_v, p, **kw, **extra)))
+        if n_osc == 32:
+            for name, extra in (("sparse", dict(sparsity=mask)), ("dense", {})):
+                def value_and_grad(extra=extra):
+                    hist, sens = rodas5P.solve_sensitivities(fn, y0, t_v, p, wrt_y0=False,
+                                                             **kw, **extra)
+                    return hist[:, -1].sum(), sens[:, -1, :, 0].sum(dim=1)
+                record(f"vdp_ring_n{nv}_value_and_grad_{name}", clock(value_and_grad),
+                       method="forward sensitivities (joint solve)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -742,6 +820,8 @@ def main():
             **ref,
         }
 
+    if not args.skip_reference_config:
+        line["reference_benchmarks"] = reference_benchmarks(torch, log)
     if not args.skip_uneven_stiff:
         line["uneven_stiff"] = uneven_stiff(torch, dev, O, log)
     if not args.skip_adjoint:
