@@ -484,8 +484,9 @@ def reference_benchmarks(torch, log_fn, reps=5):
     the sparse FP32, dense FP32 and dense FP64 LU; and the value and gradient
     of the summed final state with respect to every trajectory's parameter by
     forward sensitivities (the reference's jax.value_and_grad), sparse and
-    dense. rtol 1e-6, atol 1e-8 throughout."""
-    from menodax_amd import rodas5P, systems as S, tsit5
+    dense. rtol 1e-6, atol 1e-8 throughout. The dense rings are also run with
+    the autotuner's choice (reported next to the default's time)."""
+    from menodax_amd import autotune, rodas5P, systems as S, tsit5
 
     dev = torch.device("cuda", 0)
 
@@ -523,8 +524,16 @@ def reference_benchmarks(torch, log_fn, reps=5):
                  ("dense_fp32", dict(lu_precision="fp32")),
                  ("dense_fp64", dict(lu_precision="fp64")))
         for name, extra in cases:
-            record(f"vdp_ring_n{nv}_{name}",
-                   clock(lambda: rodas5P.solve(fn, y0, t_v, p, **kw, **extra)))
+            ms = clock(lambda: rodas5P.solve(fn, y0, t_v, p, **kw, **extra))
+            tuned = {}
+            if name.startswith("dense"):
+                # the autotuner's placement, workgroup size and launch shape
+                # (tuning, like compilation, outside the timed calls)
+                opts = autotune.tuned_options(fn, y0, t_v, p, reps=1, blocks=(16, 32, 64),
+                                              **kw, **extra)
+                tuned = {"default_ms": ms, "tuned": opts}
+                ms = clock(lambda: rodas5P.solve(fn, y0, t_v, p, **kw, **extra, **opts))
+            record(f"vdp_ring_n{nv}_{name}", ms, **tuned)
         if n_osc == 32:
             for name, extra in (("sparse", dict(sparsity=mask)), ("dense", {})):
                 def value_and_grad(extra=extra):
