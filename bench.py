@@ -463,6 +463,8 @@ def tsit5_adjoint_line(torch, log_fn, n, ckpt_every):
 REF_PUBLISHED_MS = {
     "lorenz_tsit5_n100000": (93.62, "nonstiff_lorenz_scaling/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:11"),
     "lorenz_tsit5_n1000": (3.26, "nonstiff_lorenz_scaling/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:7"),
+    "vdp_ring_nonstiff_n64_tsit5": (2.095, "nonstiff_vdp_dimensionality/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:10"),
+    "vdp_ring_nonstiff_n128_tsit5": (5.284, "nonstiff_vdp_dimensionality/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:11"),
     "vdp_ring_n32_sparse_fp32": (1.60, "stiff_vdp_dimensionality/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:9"),
     "vdp_ring_n32_dense_fp32": (43.44, "stiff_vdp_dimensionality/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:20"),
     "vdp_ring_n32_dense_fp64": (52.98, "stiff_vdp_dimensionality/results-NVIDIA_GeForce_RTX_4070_SUPER.csv:31"),
@@ -479,7 +481,8 @@ def reference_benchmarks(torch, log_fn, reps=5):
     the public API on device-resident inputs (one warm-up, then the mean of
     ``reps`` synchronised calls, as its time_blocked does): Lorenz Tsit5 on
     identical trajectories (y0 = (1000, -500, 500), rho = 0.5, t in [0, 5],
-    first_step 1e-4); the stiff Van der Pol ring (mu = 100, d = 10, 1,000
+    first_step 1e-4); Tsit5 on the non-stiff Van der Pol ring (mu = 1, 64 and
+    128 variables, 1,000 identical trajectories); the stiff Van der Pol ring (mu = 100, d = 10, 1,000
     identical trajectories, saves at 0, 0.25, ..., 1, first_step 1e-4) with
     the sparse FP32, dense FP32 and dense FP64 LU; and the value and gradient
     of the summed final state with respect to every trajectory's parameter by
@@ -514,6 +517,13 @@ def reference_benchmarks(torch, log_fn, reps=5):
         p = torch.tensor(np.tile(S.LORENZ_PARAMS, (n, 1)), device=dev)
         record(f"lorenz_tsit5_n{n}", clock(lambda: tsit5.solve(S.lorenz, y0, t_l, p, **kw)))
     t_v = torch.tensor(S.VDP_TIMES, device=dev)
+    for n_osc in (32, 64):
+        # the non-stiff ring (mu = 1) under Tsit5
+        fn, _ = S.vdp_lattice(n_osc, mu=1.0)
+        y0n, pn = S.vdp_scenario(n_osc, 1000, divergence=0.0)
+        y0, p = torch.tensor(y0n, device=dev), torch.tensor(pn, device=dev)
+        record(f"vdp_ring_nonstiff_n{2 * n_osc}_tsit5",
+               clock(lambda: tsit5.solve(fn, y0, t_v, p, **kw)))
     for n_osc in (16, 32):
         fn, _ = S.vdp_lattice(n_osc)
         mask = S.vdp_sparsity(n_osc)
