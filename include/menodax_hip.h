@@ -105,6 +105,26 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
                          int32_t* loop_steps, double* hook_out, int32_t hook_size,
                          void* workspace, int64_t workspace_bytes);
 
+/* mdx_solve_launch with a claim order for a persistent (refill) kernel: its
+ * lanes take trajectories order[0], order[1], ... as lanes free up, instead of
+ * 0, 1, .... `order` is a device array of N int64, a permutation of 0..N-1
+ * (the host checks that; an entry outside [0, N) is skipped on the device
+ * rather than read past the arrays); each trajectory's results do not depend
+ * on it. A caller that can rank its trajectories by cost -- the step counts of
+ * an earlier solve of the same or a similar ensemble -- claims the expensive
+ * ones first and shortens the ensemble's tail. NULL is mdx_solve_launch; a
+ * one-lane-per-trajectory kernel takes no order. No counterpart in the
+ * reference, whose launch gives every trajectory a thread of its own
+ * (modax/rodas5P.py:1010-1021). */
+int32_t mdx_solve_launch_ordered(int64_t handle, void* stream, int32_t grid, int64_t n_traj,
+                                 int32_t n_vars, int32_t n_params, int32_t n_save, int32_t layout,
+                                 const double* y0, const double* times, const double* params,
+                                 double dt0, double rtol, double atol, int32_t max_steps,
+                                 const double* weights, double* hist, int32_t* accepted,
+                                 int32_t* rejected, int32_t* loop_steps, double* hook_out,
+                                 int32_t hook_size, void* workspace, int64_t workspace_bytes,
+                                 const int64_t* order);
+
 /* Device memory mdx_adjoint_launch needs as `workspace`: per trajectory,
  * max_ckpt checkpoints of n_vars + 8 doubles and ckpt_every step records of
  * n_vars + 4 doubles. */

@@ -127,3 +127,29 @@ def allocate_outputs(n, n_save, n_vars, dev):
         rejected=torch.empty(n, dtype=torch.int32, device=dev),
         loop=torch.empty(n, dtype=torch.int32, device=dev),
     )
+
+
+def work_order(order, n, dev):
+    """A claim order for the persistent launch as an int64 device tensor,
+    checked to be a permutation of the trajectory indices 0..n-1 (the kernel
+    reads it without bounds beyond skipping out-of-range entries)."""
+    import torch
+
+    o = torch.as_tensor(order, dtype=torch.int64, device=dev).reshape(-1).contiguous()
+    if o.numel() != n:
+        raise ValueError(f"work_order must hold {n} trajectory indices; got {o.numel()}")
+    if n and not torch.equal(torch.sort(o).values, torch.arange(n, dtype=torch.int64, device=dev)):
+        raise ValueError("work_order must be a permutation of the trajectory indices 0..N-1")
+    return o
+
+
+def cost_order(attempted):
+    """The trajectories by decreasing cost (attempted steps of an earlier
+    solve, e.g. ``stats["accepted_steps"] + stats["rejected_steps"]``): a
+    ``work_order`` that claims the longest trajectories first."""
+    import numpy as np
+    import torch
+
+    if isinstance(attempted, torch.Tensor):
+        return torch.argsort(attempted.to(torch.int64), descending=True, stable=True)
+    return np.argsort(-np.asarray(attempted, dtype=np.int64), kind="stable")

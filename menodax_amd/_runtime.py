@@ -91,6 +91,8 @@ def lib():
             vp, vp, vp, vp, vp, i32, vp, i64,
         ]
         L.mdx_solve_launch.restype = i32
+        L.mdx_solve_launch_ordered.argtypes = L.mdx_solve_launch.argtypes + [vp]
+        L.mdx_solve_launch_ordered.restype = i32
         L.mdx_adjoint_workspace_bytes.argtypes = [i64, i32, i32, i32]
         L.mdx_adjoint_workspace_bytes.restype = i64
         L.mdx_adjoint_launch.argtypes = [
@@ -282,14 +284,14 @@ def adjoint_launch(handle, stream, n_traj, n_vars, n_params, n_save, layout, y0,
 
 def solve_launch(handle, stream, grid, n_traj, n_vars, n_params, n_save, layout, y0, times, params,
                  dt0, rtol, atol, max_steps, weights, hist, accepted, rejected, loop_steps,
-                 hook_out=0, hook_size=0, workspace=0, workspace_bytes=0):
-    """Launch through ``mdx_solve_launch``; pointers are plain integers."""
-    _check(
-        lib().mdx_solve_launch(
-            handle, stream, int(grid), int(n_traj), int(n_vars), int(n_params), int(n_save),
+                 hook_out=0, hook_size=0, workspace=0, workspace_bytes=0, order=0):
+    """Launch through ``mdx_solve_launch`` (``mdx_solve_launch_ordered`` when a
+    claim ``order`` pointer is given); pointers are plain integers."""
+    args = (handle, stream, int(grid), int(n_traj), int(n_vars), int(n_params), int(n_save),
             int(layout), y0, times, params, float(dt0), float(rtol), float(atol), int(max_steps),
             weights or None, hist, accepted, rejected, loop_steps, hook_out or None,
-            int(hook_size), workspace or None, int(workspace_bytes),
-        ),
-        "mdx_solve_launch",
-    )
+            int(hook_size), workspace or None, int(workspace_bytes))
+    if order:
+        _check(lib().mdx_solve_launch_ordered(*args, order), "mdx_solve_launch_ordered")
+    else:
+        _check(lib().mdx_solve_launch(*args), "mdx_solve_launch")

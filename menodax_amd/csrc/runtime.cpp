@@ -41,6 +41,7 @@ struct SolveArgs {
   int32_t* loop;
   double* hook_out;
   unsigned long long* queue;
+  const int64_t* order;
   double dt0, rtol, atol;
   int64_t n_traj;
   int64_t y0_ts, y0_vs;
@@ -51,7 +52,7 @@ struct SolveArgs {
   int32_t max_steps;
   int32_t n_save;
 };
-static_assert(sizeof(SolveArgs) == 10 * 8 + 3 * 8 + 8 + 11 * 8 + 8, "SolveArgs layout");
+static_assert(sizeof(SolveArgs) == 11 * 8 + 3 * 8 + 8 + 11 * 8 + 8, "SolveArgs layout");
 
 // Mirrors mdx::AdjointArgs in kernels/mdx_common.hpp (both adjoint kernels).
 struct AdjointArgs {
@@ -270,6 +271,20 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
                          double* hist, int32_t* accepted, int32_t* rejected,
                          int32_t* loop_steps, double* hook_out, int32_t hook_size,
                          void* workspace, int64_t workspace_bytes) {
+  return mdx_solve_launch_ordered(handle, stream, grid, n_traj, n_vars, n_params, n_save, layout,
+                                  y0, times, params, dt0, rtol, atol, max_steps, weights, hist,
+                                  accepted, rejected, loop_steps, hook_out, hook_size, workspace,
+                                  workspace_bytes, nullptr);
+}
+
+int32_t mdx_solve_launch_ordered(int64_t handle, void* stream, int32_t grid, int64_t n_traj,
+                                 int32_t n_vars, int32_t n_params, int32_t n_save, int32_t layout,
+                                 const double* y0, const double* times, const double* params,
+                                 double dt0, double rtol, double atol, int32_t max_steps,
+                                 const double* weights, double* hist, int32_t* accepted,
+                                 int32_t* rejected, int32_t* loop_steps, double* hook_out,
+                                 int32_t hook_size, void* workspace, int64_t workspace_bytes,
+                                 const int64_t* order) {
   Kernel* k = nullptr;
   if (int32_t rc = lookup(handle, &k)) return rc;
   if (n_traj < 0 || n_vars < 1 || n_params < 0 || n_save < 1)
@@ -290,6 +305,8 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
   if (grid > 0 && !k->persistent && (int64_t)grid * k->block < n_traj)
     return fail(1, "mdx_solve_launch: a one-lane-per-trajectory launch needs grid >= "
                    "ceil(n_traj / block)");
+  if (order && !k->persistent)
+    return fail(1, "mdx_solve_launch_ordered: a claim order needs a persistent (refill) kernel");
   if (k->persistent && (!workspace || workspace_bytes < 8))
     return fail(1, "mdx_solve_launch: a persistent kernel needs a workspace of at least 8 bytes");
   if (n_traj == 0) return 0;
@@ -306,6 +323,7 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
   a.loop = loop_steps;
   a.hook_out = hook_out;
   a.queue = (unsigned long long*)workspace;
+  a.order = order;
   a.dt0 = dt0;
   a.rtol = rtol;
   a.atol = atol;

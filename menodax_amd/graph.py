@@ -30,7 +30,7 @@ class CapturedSolve:
     def __init__(self, ode_fn, y0, t_span, params, *, rtol=1e-8, atol=1e-10, first_step=None,
                  max_steps=100000, pcoeff=0.0, icoeff=1.0, dcoeff=0.0, lu_precision="fp32",
                  trajectories_per_block=None, sparsity=None, ordering="amd", tf_index=None,
-                 max_registers=None, refill=None, placement=None):
+                 max_registers=None, refill=None, placement=None, work_order=None):
         import torch
 
         self._torch = torch
@@ -53,6 +53,10 @@ class CapturedSolve:
         self._times = _common.as_f64_device(t_span, dev).contiguous().clone()
         self._kw = dict(dt0=_common.initial_step(first_step), rtol=rtol, atol=atol,
                         max_steps=max_steps, refill=refill)
+        if work_order is not None:
+            # the claim order lives in a buffer the graph reads: set_work_order
+            # replaces its contents between replays
+            self._kw["order"] = _common.work_order(work_order, n, dev).clone()
         # eager launch: loads the module, allocates outputs and workspace
         self._out = launch(self.built, self._y0.t(), self._times, self._p.t(), n, **self._kw)
         self._ws = self._out._keep[-1]
@@ -92,6 +96,15 @@ class CapturedSolve:
             hist = hist.cpu().numpy()
             stats = {k: v.cpu().numpy() for k, v in stats.items()}
         return (hist, stats) if return_stats else hist
+
+    def set_work_order(self, order):
+        """Replace the claim order of a solve built with ``work_order`` (a
+        permutation of the trajectories, e.g. ``_common.cost_order`` of the
+        last run's step counts); takes effect from the next ``run``."""
+        if "order" not in self._kw:
+            raise ValueError("this CapturedSolve was built without a work_order")
+        buf = self._kw["order"]
+        buf.copy_(_common.work_order(order, self.n, buf.device))
 
     @property
     def graph(self):

@@ -31,6 +31,7 @@ struct SolveArgs {
   int* loop;
   double* hook_out;       // (traj, hook_size): save-hook accumulators
   u64* queue;             // persistent refill: next unclaimed trajectory
+  const i64* order;       // persistent refill: claim order (a permutation), or null
   double dt0, rtol, atol;
   i64 n_traj;
   i64 y0_ts, y0_vs;

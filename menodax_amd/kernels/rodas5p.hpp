@@ -1030,9 +1030,14 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p_
     const mdx::i64 got = mdx::wave_claim(a.queue, want);
     if (want) {
       if (got < a.n_traj) {
-        i = got;
-        p.load(a, i);
-        init(s, a, i, p);
+        // the claim order, when given, maps the claim to a trajectory; an
+        // entry out of range is skipped and the lane claims again
+        const mdx::i64 j = a.order ? a.order[got] : got;
+        if (j >= 0 && j < a.n_traj) {
+          i = j;
+          p.load(a, i);
+          init(s, a, i, p);
+        }
       } else {
         exhausted = true;
       }

@@ -482,9 +482,14 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_tsit5_refill(mdx::So
     const mdx::i64 got = mdx::wave_claim(a.queue, want);
     if (want) {
       if (got < a.n_traj) {
-        i = got;
-        p.load(a, i);
-        init(s, a, i);
+        // the claim order, when given, maps the claim to a trajectory; an
+        // entry out of range is skipped and the lane claims again
+        const mdx::i64 j = a.order ? a.order[got] : got;
+        if (j >= 0 && j < a.n_traj) {
+          i = j;
+          p.load(a, i);
+          init(s, a, i);
+        }
       } else {
         exhausted = true;
       }

@@ -80,6 +80,24 @@ def test_a_persistent_kernel_needs_its_workspace(handles):
         _launch(handles[1])
 
 
+def test_a_claim_order_needs_a_persistent_kernel(handles):
+    with pytest.raises(_runtime.NativeError, match="needs a persistent"):
+        _launch(handles[0], order=8)
+
+
+def test_work_order_must_be_a_permutation():
+    import torch
+
+    from menodax_amd import _common
+
+    cpu = torch.device("cpu")
+    assert _common.work_order([2, 0, 1], 3, cpu).tolist() == [2, 0, 1]
+    for bad in ([0, 0, 1], [0, 1], [0, 1, 3]):
+        with pytest.raises(ValueError, match="work_order"):
+            _common.work_order(bad, 3, cpu)
+    assert _common.cost_order([5, 9, 1, 9]).tolist() == [1, 3, 0, 2]
+
+
 def test_an_empty_ensemble_launches_nothing(handles):
     _launch(handles[0], n_traj=0)  # returns before any device call
 

@@ -44,3 +44,20 @@ def test_new_inputs_through_the_captured_buffers():
     np.testing.assert_array_equal(second, rodas5P.solve(S.robertson, y0b, t, pb, **kw))
     again = cs.run(y0=y0, params=p)
     np.testing.assert_array_equal(again, first)
+
+
+def test_a_claim_order_set_between_replays():
+    from menodax_amd import _common
+
+    y0, p = S.robertson_lognormal_ensemble(4096, seed=4)
+    t = S.robertson_benchmark_times()
+    kw = dict(rtol=1e-6, atol=1e-8)
+    eager, est = rodas5P.solve(S.robertson, y0, t, p, return_stats=True, **kw)
+    cs = CapturedSolve(S.robertson, y0, t, p, refill=True, work_order=np.arange(4096), **kw)
+    hist, st = cs.run(return_stats=True)
+    np.testing.assert_array_equal(hist, eager)
+    cs.set_work_order(_common.cost_order(st["accepted_steps"] + st["rejected_steps"]))
+    for _ in range(2):
+        np.testing.assert_array_equal(cs.run(), eager)
+    with pytest.raises(ValueError, match="permutation"):
+        cs.set_work_order(np.zeros(4096, dtype=np.int64))
