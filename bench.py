@@ -202,6 +202,24 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64, variants=UNEVEN_VAR
                     "bit_identical": bool(np.array_equal(o2.ensemble_view().cpu().numpy(),
                                                          hists[key]))}
                 log_fn(f"uneven stiff fp32_refill cost-ordered: {ms2:.1f} ms")
+
+                # a first solve that ranks its trajectories itself: a pilot of
+                # every trajectory, the device sort, then the ordered solve,
+                # all inside the timed region
+                def run_piloted(built=built, outs=outs, ws=ws):
+                    return rodas5P.launch(built, y0_t, t_t, p_t, n, dt0=0.0, rtol=RTOL,
+                                          atol=ATOL, max_steps=100000, refill=True,
+                                          outputs=outs, workspace=ws, order="pilot")
+
+                o3, wall3, _ = time_solves(run_piloted, 2, 0, torch)
+                ms3 = wall3 / 2 * 1e3
+                res["fp32_refill_piloted"] = {
+                    "ms": ms3, "trajectories_per_s": n / (ms3 / 1e3),
+                    "order": f"a {rodas5P.PILOT_STEPS}-step pilot of every trajectory ranks them "
+                             "by steps left; pilot and sort inside the timed region",
+                    "bit_identical": bool(np.array_equal(o3.ensemble_view().cpu().numpy(),
+                                                         hists[key]))}
+                log_fn(f"uneven stiff fp32_refill piloted: {ms3:.1f} ms")
     if not {"fp32_refill", "fp64_plain"} <= set(res):
         return {"variants": res}
     # the autotuner over placement x workgroup size on the whole ensemble, in
@@ -249,6 +267,8 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64, variants=UNEVEN_VAR
                     "rtol 1e-6 atol 1e-8, fp64 state, sparse LU (AMD)",
         **res,
         "fp32_refill_over_fp64_plain": res["fp64_plain"]["ms"] / res["fp32_refill"]["ms"],
+        "fp32_refill_piloted_over_fp64_plain": (res["fp64_plain"]["ms"]
+                                                / res["fp32_refill_piloted"]["ms"]),
         "refill_bit_identical_to_plain": same,
         "attempted_steps": {"p1": float(np.percentile(steps, 1)),
                             "median": float(np.median(steps)),

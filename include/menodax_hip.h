@@ -125,6 +125,22 @@ int32_t mdx_solve_launch_ordered(int64_t handle, void* stream, int32_t grid, int
                                  int32_t hook_size, void* workspace, int64_t workspace_bytes,
                                  const int64_t* order);
 
+/* A cost pilot for a claim order: the one-lane-per-trajectory kernel runs each
+ * trajectory for at most `steps` steps and writes cost[i], the steps left at
+ * its current step size ((t_end - t) / dt; 0 when it finished), instead of its
+ * remaining saves and counters. Trajectories by decreasing cost are a claim
+ * order for mdx_solve_launch_ordered (a first solve's estimate of what an
+ * earlier solve's step counts give exactly). hist and the counters receive the
+ * pilot's partial results and are scratch to the caller; the other buffers as
+ * mdx_solve_launch. The kernel must not be persistent or run a save hook. No
+ * counterpart in the reference. */
+int32_t mdx_cost_pilot_launch(int64_t handle, void* stream, int64_t n_traj, int32_t n_vars,
+                              int32_t n_params, int32_t n_save, int32_t layout, const double* y0,
+                              const double* times, const double* params, double dt0, double rtol,
+                              double atol, int32_t steps, const double* weights, double* hist,
+                              int32_t* accepted, int32_t* rejected, int32_t* loop_steps,
+                              double* cost);
+
 /* Device memory mdx_adjoint_launch needs as `workspace`: per trajectory,
  * max_ckpt checkpoints of n_vars + 8 doubles and ckpt_every step records of
  * n_vars + 4 doubles. */

@@ -143,6 +143,16 @@ def work_order(order, n, dev):
     return o
 
 
+def work_order_arg(value, n, dev):
+    """``solve``'s ``work_order`` as ``launch`` takes it: None, ``"pilot"``, or
+    a checked permutation tensor."""
+    if value is None or (isinstance(value, str) and value == "pilot"):
+        return value
+    if isinstance(value, str):
+        raise ValueError(f"work_order must be a permutation or 'pilot', got {value!r}")
+    return work_order(value, n, dev)
+
+
 def cost_order(attempted):
     """The trajectories by decreasing cost (attempted steps of an earlier
     solve, e.g. ``stats["accepted_steps"] + stats["rejected_steps"]``): a

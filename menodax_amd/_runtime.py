@@ -93,6 +93,11 @@ def lib():
         L.mdx_solve_launch.restype = i32
         L.mdx_solve_launch_ordered.argtypes = L.mdx_solve_launch.argtypes + [vp]
         L.mdx_solve_launch_ordered.restype = i32
+        L.mdx_cost_pilot_launch.argtypes = [
+            i64, vp, i64, i32, i32, i32, i32, vp, vp, vp, dbl, dbl, dbl, i32, vp,
+            vp, vp, vp, vp, vp,
+        ]
+        L.mdx_cost_pilot_launch.restype = i32
         L.mdx_adjoint_workspace_bytes.argtypes = [i64, i32, i32, i32]
         L.mdx_adjoint_workspace_bytes.restype = i64
         L.mdx_adjoint_launch.argtypes = [
@@ -295,3 +300,16 @@ def solve_launch(handle, stream, grid, n_traj, n_vars, n_params, n_save, layout,
         _check(lib().mdx_solve_launch_ordered(*args, order), "mdx_solve_launch_ordered")
     else:
         _check(lib().mdx_solve_launch(*args), "mdx_solve_launch")
+
+
+def cost_pilot_launch(handle, stream, n_traj, n_vars, n_params, n_save, layout, y0, times, params,
+                      dt0, rtol, atol, steps, weights, hist, accepted, rejected, loop_steps, cost):
+    """Launch through ``mdx_cost_pilot_launch``; pointers are plain integers."""
+    _check(
+        lib().mdx_cost_pilot_launch(
+            handle, stream, int(n_traj), int(n_vars), int(n_params), int(n_save), int(layout),
+            y0, times, params, float(dt0), float(rtol), float(atol), int(steps),
+            weights or None, hist, accepted, rejected, loop_steps, cost,
+        ),
+        "mdx_cost_pilot_launch",
+    )

@@ -42,6 +42,7 @@ struct SolveArgs {
   double* hook_out;
   unsigned long long* queue;
   const int64_t* order;
+  double* cost;
   double dt0, rtol, atol;
   int64_t n_traj;
   int64_t y0_ts, y0_vs;
@@ -52,7 +53,7 @@ struct SolveArgs {
   int32_t max_steps;
   int32_t n_save;
 };
-static_assert(sizeof(SolveArgs) == 11 * 8 + 3 * 8 + 8 + 11 * 8 + 8, "SolveArgs layout");
+static_assert(sizeof(SolveArgs) == 12 * 8 + 3 * 8 + 8 + 11 * 8 + 8, "SolveArgs layout");
 
 // Mirrors mdx::AdjointArgs in kernels/mdx_common.hpp (both adjoint kernels).
 struct AdjointArgs {
@@ -277,6 +278,16 @@ int32_t mdx_solve_launch(int64_t handle, void* stream, int32_t grid, int64_t n_t
                                   workspace_bytes, nullptr);
 }
 
+namespace {
+int32_t launch_impl(int64_t handle, void* stream, int32_t grid, int64_t n_traj, int32_t n_vars,
+                    int32_t n_params, int32_t n_save, int32_t layout, const double* y0,
+                    const double* times, const double* params, double dt0, double rtol,
+                    double atol, int32_t max_steps, const double* weights, double* hist,
+                    int32_t* accepted, int32_t* rejected, int32_t* loop_steps, double* hook_out,
+                    int32_t hook_size, void* workspace, int64_t workspace_bytes,
+                    const int64_t* order, double* cost);
+}  // namespace
+
 int32_t mdx_solve_launch_ordered(int64_t handle, void* stream, int32_t grid, int64_t n_traj,
                                  int32_t n_vars, int32_t n_params, int32_t n_save, int32_t layout,
                                  const double* y0, const double* times, const double* params,
@@ -285,6 +296,32 @@ int32_t mdx_solve_launch_ordered(int64_t handle, void* stream, int32_t grid, int
                                  int32_t* rejected, int32_t* loop_steps, double* hook_out,
                                  int32_t hook_size, void* workspace, int64_t workspace_bytes,
                                  const int64_t* order) {
+  return launch_impl(handle, stream, grid, n_traj, n_vars, n_params, n_save, layout, y0, times,
+                     params, dt0, rtol, atol, max_steps, weights, hist, accepted, rejected,
+                     loop_steps, hook_out, hook_size, workspace, workspace_bytes, order, nullptr);
+}
+
+int32_t mdx_cost_pilot_launch(int64_t handle, void* stream, int64_t n_traj, int32_t n_vars,
+                              int32_t n_params, int32_t n_save, int32_t layout, const double* y0,
+                              const double* times, const double* params, double dt0, double rtol,
+                              double atol, int32_t steps, const double* weights, double* hist,
+                              int32_t* accepted, int32_t* rejected, int32_t* loop_steps,
+                              double* cost) {
+  if (!cost) return fail(1, "mdx_cost_pilot_launch: null cost buffer");
+  if (steps < 1) return fail(1, "mdx_cost_pilot_launch: steps must be positive");
+  return launch_impl(handle, stream, 0, n_traj, n_vars, n_params, n_save, layout, y0, times,
+                     params, dt0, rtol, atol, steps, weights, hist, accepted, rejected,
+                     loop_steps, nullptr, 0, nullptr, 0, nullptr, cost);
+}
+
+namespace {
+int32_t launch_impl(int64_t handle, void* stream, int32_t grid, int64_t n_traj, int32_t n_vars,
+                    int32_t n_params, int32_t n_save, int32_t layout, const double* y0,
+                    const double* times, const double* params, double dt0, double rtol,
+                    double atol, int32_t max_steps, const double* weights, double* hist,
+                    int32_t* accepted, int32_t* rejected, int32_t* loop_steps, double* hook_out,
+                    int32_t hook_size, void* workspace, int64_t workspace_bytes,
+                    const int64_t* order, double* cost) {
   Kernel* k = nullptr;
   if (int32_t rc = lookup(handle, &k)) return rc;
   if (n_traj < 0 || n_vars < 1 || n_params < 0 || n_save < 1)
@@ -307,6 +344,9 @@ int32_t mdx_solve_launch_ordered(int64_t handle, void* stream, int32_t grid, int
                    "ceil(n_traj / block)");
   if (order && !k->persistent)
     return fail(1, "mdx_solve_launch_ordered: a claim order needs a persistent (refill) kernel");
+  if (cost && (k->persistent || k->hook))
+    return fail(1, "mdx_cost_pilot_launch: a cost pilot runs a one-lane-per-trajectory kernel "
+                   "without a save hook");
   if (k->persistent && (!workspace || workspace_bytes < 8))
     return fail(1, "mdx_solve_launch: a persistent kernel needs a workspace of at least 8 bytes");
   if (n_traj == 0) return 0;
@@ -324,6 +364,7 @@ int32_t mdx_solve_launch_ordered(int64_t handle, void* stream, int32_t grid, int
   a.hook_out = hook_out;
   a.queue = (unsigned long long*)workspace;
   a.order = order;
+  a.cost = cost;
   a.dt0 = dt0;
   a.rtol = rtol;
   a.atol = atol;
@@ -376,6 +417,7 @@ int32_t mdx_solve_launch_ordered(int64_t handle, void* stream, int32_t grid, int
   if (e != hipSuccess) return hip_fail(e, ("hipModuleLaunchKernel(" + k->entry + ")").c_str());
   return 0;
 }
+}  // namespace
 
 int64_t mdx_adjoint_workspace_bytes(int64_t n_traj, int32_t n_vars, int32_t ckpt_every,
                                     int32_t max_ckpt) {

@@ -53,6 +53,9 @@ class CapturedSolve:
         self._times = _common.as_f64_device(t_span, dev).contiguous().clone()
         self._kw = dict(dt0=_common.initial_step(first_step), rtol=rtol, atol=atol,
                         max_steps=max_steps, refill=refill)
+        if isinstance(work_order, str):
+            raise ValueError("a captured solve takes an explicit work_order (a permutation); "
+                             "a pilot's device sort is not captured")
         if work_order is not None:
             # the claim order lives in a buffer the graph reads: set_work_order
             # replaces its contents between replays

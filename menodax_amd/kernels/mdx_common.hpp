@@ -32,6 +32,7 @@ struct SolveArgs {
   double* hook_out;       // (traj, hook_size): save-hook accumulators
   u64* queue;             // persistent refill: next unclaimed trajectory
   const i64* order;       // persistent refill: claim order (a permutation), or null
+  double* cost;           // cost pilot: work left per trajectory, written instead of saves
   double dt0, rtol, atol;
   i64 n_traj;
   i64 y0_ts, y0_vs;

@@ -963,6 +963,12 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
 __device__ __forceinline__ void finish(State& s, const PA& p, const SolveArgs& a, i64 i) {
   MDX_LANE();
   (void)p;
+  if (a.cost) {
+    // a cost pilot (max_steps is its step budget): the steps left at the
+    // current step size instead of the saves and counters, 0 once finished
+    a.cost[i] = (s.save_idx < a.n_save && s.t < s.tf) ? (s.tf - s.t) / s.dt : 0.0;
+    return;
+  }
   while (s.save_idx < a.n_save) {
 #if MDX_SAVE_HISTORY
     double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;

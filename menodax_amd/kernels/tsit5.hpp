@@ -443,6 +443,12 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
 #endif
 
 __device__ __forceinline__ void finish(State& s, const SolveArgs& a, i64 i) {
+  if (a.cost) {
+    // a cost pilot (max_steps is its step budget): the steps left at the
+    // current step size instead of the saves and counters, 0 once finished
+    a.cost[i] = (s.save_idx < a.n_save && s.t < s.tf) ? (s.tf - s.t) / s.dt : 0.0;
+    return;
+  }
   const double nan = __builtin_nan("");
   while (s.save_idx < a.n_save) {
     double* __restrict__ row = a.hist + i * a.h_ts + (i64)s.save_idx * a.h_ss;

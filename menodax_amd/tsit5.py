@@ -96,7 +96,7 @@ def solve(
         raise ValueError("t_span must be a non-empty 1-D array of save times")
     launch_kw = dict(dt0=_common.initial_step(first_step), rtol=rtol, atol=atol,
                      max_steps=max_steps, refill=refill, stream=stream,
-                     order=None if work_order is None else _common.work_order(work_order, n, dev))
+                     order=_common.work_order_arg(work_order, n, dev))
     wrt_y0, wrt_params = _autodiff.wanted(y0, params)
     if (wrt_y0 or wrt_params) and adjoint:
         _check_backend(n_vars, backend)

@@ -85,6 +85,18 @@ def test_a_claim_order_needs_a_persistent_kernel(handles):
         _launch(handles[0], order=8)
 
 
+def test_a_cost_pilot_runs_a_one_lane_kernel(handles):
+    kw = dict(n_traj=4, n_vars=3, n_params=3, n_save=5, layout=1, y0=8, times=8, params=8,
+              dt0=0.0, rtol=1e-6, atol=1e-8, steps=10, weights=8, hist=8, accepted=8,
+              rejected=8, loop_steps=8, cost=8)
+    with pytest.raises(_runtime.NativeError, match="one-lane-per-trajectory"):
+        _runtime.cost_pilot_launch(handles[1], None, **kw)
+    with pytest.raises(_runtime.NativeError, match="steps must be positive"):
+        _runtime.cost_pilot_launch(handles[0], None, **dict(kw, steps=0))
+    with pytest.raises(_runtime.NativeError, match="null cost"):
+        _runtime.cost_pilot_launch(handles[0], None, **dict(kw, cost=0))
+
+
 def test_work_order_must_be_a_permutation():
     import torch
 
@@ -96,6 +108,9 @@ def test_work_order_must_be_a_permutation():
         with pytest.raises(ValueError, match="work_order"):
             _common.work_order(bad, 3, cpu)
     assert _common.cost_order([5, 9, 1, 9]).tolist() == [1, 3, 0, 2]
+    assert _common.work_order_arg("pilot", 3, cpu) == "pilot"
+    with pytest.raises(ValueError, match="pilot"):
+        _common.work_order_arg("fastest", 3, cpu)
 
 
 def test_an_empty_ensemble_launches_nothing(handles):

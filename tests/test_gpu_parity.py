@@ -270,7 +270,8 @@ def test_every_placement_is_bit_exact(placement, lu):
 
 def test_a_claim_order_changes_no_bit():
     """The persistent launch handing trajectories out in a given order -- the
-    costliest first, or shuffled -- computes every trajectory as the oracle does."""
+    costliest first, shuffled, or ranked by a short pilot -- computes every
+    trajectory as the oracle does."""
     from menodax_amd import _common, _sparse_direct as SD
     from menodax_amd._sparsity import normalize_sparsity
 
@@ -284,15 +285,16 @@ def test_a_claim_order_changes_no_bit():
     kw = dict(sparsity=mask, rtol=1e-6, atol=1e-8, lu_precision="fp32", return_stats=True)
     cost = _common.cost_order(rst["accepted_steps"] + rst["rejected_steps"])
     shuffled = np.random.default_rng(3).permutation(len(y0))
-    for order in (cost, shuffled):
+    for order in (cost, shuffled, "pilot"):
         hist, st = rodas5P.solve(fn, y0, t, lam, work_order=order, **kw)
         _same(hist, st, ref, rst)
     y0l, pl = S.lorenz_rho_ensemble(1024)
     tl = np.linspace(0.0, 1.0, 100)
-    hist, st = tsit5.solve(S.lorenz, y0l, tl, pl, rtol=1e-6, atol=1e-8, return_stats=True,
-                           work_order=np.random.default_rng(4).permutation(1024))
     ref, rst = O.tsit5(O.lorenz(), y0l, tl, pl, rtol=1e-6, atol=1e-8)
-    _same(hist, st, ref, rst)
+    for order in (np.random.default_rng(4).permutation(1024), "pilot"):
+        hist, st = tsit5.solve(S.lorenz, y0l, tl, pl, rtol=1e-6, atol=1e-8, return_stats=True,
+                               work_order=order)
+        _same(hist, st, ref, rst)
 
 
 def test_array_rhs_stage_evaluations_are_bit_identical():
