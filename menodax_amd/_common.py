@@ -22,6 +22,20 @@ def _torch():
     return torch
 
 
+def eager_only(fn):
+    """``fn`` run eagerly when a ``torch.compile``d caller reaches it: the
+    compiler stops its graph at the call (a graph break) instead of tracing
+    into the launch, whose stream handles and device pointers exist only at
+    run time. Results are unchanged; ``fullgraph=True`` callers get the
+    compiler's error for a graph break."""
+    try:
+        import torch
+
+        return torch.compiler.disable(fn, recursive=True)
+    except (ImportError, AttributeError):
+        return fn
+
+
 def device():
     torch = _torch()
     if not torch.cuda.is_available():

@@ -213,3 +213,9 @@ def solve_sensitivities(ode_fn, y0, t_span, params, *, wrt_y0=True, wrt_params=T
         state, sens = state.cpu().numpy(), sens.cpu().numpy()
         stats = {k: v.cpu().numpy() for k, v in stats.items()}
     return (state, sens, stats) if return_stats else (state, sens)
+
+
+# Called from torch.compile'd code these run eagerly (see _common.eager_only).
+solve = _common.eager_only(solve)
+adjoint_gradients = _common.eager_only(adjoint_gradients)
+solve_sensitivities = _common.eager_only(solve_sensitivities)
