@@ -684,7 +684,8 @@ def main():
 
     results = {}
     for lu in (("fp32", "fp64") if args.lu == "both" else (args.lu,)):
-        built = _kernels.rodas5p_kernel(S.robertson, 3, 3, _kernels.KernelOptions(lu_precision=lu))
+        built = _kernels.rodas5p_kernel(S.robertson, 3, 3, _kernels.KernelOptions(
+            lu_precision=lu, tableau_in_registers=rodas5P._tableau(None, 3, args.n, None, None)))
         info = _runtime.kernel_info(built.plain)
         use_refill = rodas5P._use_refill(built, args.n) if refill is None else refill
         outputs = rodas5P.launch(built, y0_v, times, p_v, args.n, dt0=0.0, rtol=RTOL, atol=ATOL,
@@ -747,6 +748,7 @@ def main():
             "n_save": int(times_np.size),
             "lu_precision": "fp32",
             "kernel": "refill" if r32["refill"] else "one lane per trajectory",
+            "tableau_in_registers": bool(r32["built"].source.count("#define MDX_TAB_REGS 1")),
         },
         "accepted_steps_per_s": r32["accepted"] / (r32["ms_per_step"] / 1e3),
         "fp64_lu": {
@@ -857,7 +859,8 @@ def main():
         tr = torch.tensor(S.ROBERTSON_TIMES, device=dev)
         ref = {}
         for lu in ("fp32", "fp64"):
-            built = _kernels.rodas5p_kernel(S.robertson, 3, 3, _kernels.KernelOptions(lu_precision=lu))
+            built = _kernels.rodas5p_kernel(S.robertson, 3, 3, _kernels.KernelOptions(
+                lu_precision=lu, tableau_in_registers=rodas5P._tableau(None, 3, n_ref, None, None)))
             outs = rodas5P.launch(built, y0r.t(), tr, pr.t(), n_ref, dt0=1e-4, rtol=RTOL, atol=ATOL,
                                   max_steps=100000, refill=False)
 

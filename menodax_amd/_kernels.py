@@ -67,6 +67,10 @@ class KernelOptions:
     #: state in scratch), "lds" (rolled, with as many of the stage vectors, the
     #: stage argument and the state in LDS as fit), or None to choose by size
     placement: str | None = None
+    #: keep the Rodas5P tableau in vector registers across the step loop (the
+    #: unrolled step only): fewer instructions per step for more registers, or
+    #: None for the size rule
+    tableau_in_registers: bool | None = None
     #: forward sensitivities carried with the state (a ``SensitivitySpec``), or None
     sens: object = None
     #: an array-form right-hand side ``f(y, t, p, out)`` for the stage
@@ -79,6 +83,28 @@ class KernelOptions:
     hook_size: int = 0
     save_history: bool = True
     hook_n_save: int = 0
+
+
+#: the unrolled Rodas5P step keeps its tableau in vector registers up to this
+#: many variables: about 100 registers more (243 for Robertson), two waves per
+#: SIMD; a 4-variable step would reach into the accumulation registers
+TABLEAU_REGS_MAX_VARS = 3
+
+
+def tableau_in_registers(n_vars: int, n_traj: int, n_cu: int, placement=None,
+                         max_registers=None) -> bool:
+    """Whether a plain Rodas5P solve keeps its tableau in registers: for the
+    unrolled step of a small system when every trajectory has a lane resident
+    at two waves per SIMD, the occupancy the larger register file leaves. Then
+    the step issues about 10% fewer instructions (Robertson: 1,032 against
+    1,145, the scalar moves that rebuild each double constant at every use);
+    beyond it the lost occupancy costs more than that (262,144 Robertson
+    trajectories, FP64 LU: 1.55 ms against 1.73 ms)."""
+    if placement not in (None, "registers") or max_registers is not None:
+        return False
+    if n_vars > TABLEAU_REGS_MAX_VARS:
+        return False
+    return n_traj <= n_cu * 4 * 2 * 64
 
 
 def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None) -> dict:
@@ -284,6 +310,8 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
         "MDX_LDS_U": place["MDX_LDS_U"],
         "MDX_LDS_Y": place["MDX_LDS_Y"],
         "MDX_MAX_VGPRS": _max_vgprs(options.max_registers),
+        "MDX_TAB_REGS": 1 if (place["MDX_ROLLED"] == 0 and options.tableau_in_registers
+                              and not adjoint and spec is None) else 0,
     }
     sens_src = ""
     if options.save_hook is not None or not options.save_history:

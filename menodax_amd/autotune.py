@@ -1,7 +1,8 @@
 """Autotuning the Rodas5P launch: placement, workgroup size, launch shape.
 
-The placements (registers / rolled scratch / rolled LDS), the trajectories per
-workgroup and the two launch shapes (one lane per trajectory, persistent
+The placements (registers / rolled scratch / rolled LDS), for the unrolled
+step of a small system whether the tableau is held in registers, the
+trajectories per workgroup and the two launch shapes (one lane per trajectory, persistent
 refill) move data and work, never arithmetic, so every candidate computes the
 same bits. Which is fastest depends on the system size, the register and LDS
 footprint and the spread of step counts, and is measured rather than guessed:
@@ -43,20 +44,24 @@ class Choice:
     trajectories_per_block: int
     refill: bool
     seconds: float
+    tableau_in_registers: bool = False
 
 
 def candidates(n_vars: int, lu_precision: str, blocks=(64, 128, 256)):
-    """(placement, block) pairs worth timing: the unrolled register kernel only
-    where it can plausibly fit, the LDS placement only at block sizes where
-    something fits in LDS."""
+    """(placement, block, tableau in registers) worth timing: the unrolled
+    register kernel only where it can plausibly fit, with and without the
+    tableau in registers where that fits too, the LDS placement only at block
+    sizes where something fits in LDS."""
     out = []
     for placement in PLACEMENTS:
         if placement == "registers" and n_vars > 32:
             continue
+        tabs = (False, True) if (placement == "registers"
+                                 and n_vars <= _kernels.TABLEAU_REGS_MAX_VARS) else (False,)
         for b in blocks:
             if placement == "lds" and rodas5p_placement(n_vars, lu_precision, b, "lds")["lds_bytes"] == 0:
                 continue
-            out.append((placement, b))
+            out.extend((placement, b, tab) for tab in tabs)
     return out
 
 
@@ -89,10 +94,11 @@ def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-
     times = _common.as_f64_device(t_span, dev).contiguous()
     kw = dict(dt0=_common.initial_step(first_step), rtol=rtol, atol=atol, max_steps=max_steps)
     results, reference = [], None
-    for placement, block in candidates(n_vars, lu_precision, blocks):
+    for placement, block, tab in candidates(n_vars, lu_precision, blocks):
         built = _kernels.rodas5p_kernel(
             ode_fn, n_vars, n_params,
-            replace(base, placement=placement, trajectories_per_block=block))
+            replace(base, placement=placement, trajectories_per_block=block,
+                    tableau_in_registers=tab))
         launches = [(False, 0), (True, 0)]
         if n < n_full or shapes == "solve":
             if not _use_refill(built, n_full):
@@ -109,15 +115,17 @@ def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-
             if reference is None:
                 reference = hist
             elif not np.array_equal(hist, reference, equal_nan=True):
-                raise RuntimeError(f"placement {placement}, block {block}, refill {refill} "
-                                   "changed the result; placements must be bit-identical")
+                raise RuntimeError(f"placement {placement}, block {block}, tableau in registers "
+                                   f"{tab}, refill {refill} changed the result; placements "
+                                   "must be bit-identical")
             ws = out._keep[-1]
             t0 = time.perf_counter()
             for _ in range(reps):
                 launch(built, y0_t, times, p_t, n, refill=refill, grid=grid, outputs=out,
                        workspace=ws, **kw)
             torch.cuda.synchronize()
-            results.append(Choice(placement, block, refill, (time.perf_counter() - t0) / reps))
+            results.append(Choice(placement, block, refill, (time.perf_counter() - t0) / reps,
+                                  tab))
     best = min(results, key=lambda c: c.seconds)
     _CACHE[key] = (best, results)
     return best, results
@@ -127,4 +135,4 @@ def tuned_options(ode_fn, y0, t_span, params, **kw) -> dict:
     """``solve`` keyword arguments for the fastest candidate (see ``tune``)."""
     best, _ = tune(ode_fn, y0, t_span, params, **kw)
     return {"placement": best.placement, "trajectories_per_block": best.trajectories_per_block,
-            "refill": best.refill}
+            "refill": best.refill, "tableau_in_registers": best.tableau_in_registers}

@@ -16,7 +16,7 @@ Replays give the same bits as the eager solve (tests/test_gpu_graph.py).
 from __future__ import annotations
 
 from . import _common, _kernels
-from .rodas5P import launch
+from .rodas5P import _tableau, launch
 from ._kernels import KernelOptions, trajectories_per_block_or_default
 from ._sparsity import normalize_sparsity
 
@@ -30,7 +30,8 @@ class CapturedSolve:
     def __init__(self, ode_fn, y0, t_span, params, *, rtol=1e-8, atol=1e-10, first_step=None,
                  max_steps=100000, pcoeff=0.0, icoeff=1.0, dcoeff=0.0, lu_precision="fp32",
                  trajectories_per_block=None, sparsity=None, ordering="amd", tf_index=None,
-                 max_registers=None, refill=None, placement=None, work_order=None):
+                 max_registers=None, refill=None, placement=None, work_order=None,
+                 tableau_in_registers=None):
         import torch
 
         self._torch = torch
@@ -44,7 +45,9 @@ class CapturedSolve:
             trajectories_per_block=trajectories_per_block_or_default(trajectories_per_block),
             sparsity=sparsity, ordering=ordering,
             tf_index=-1 if tf_index is None else int(tf_index), max_registers=max_registers,
-            placement=placement)
+            placement=placement,
+            tableau_in_registers=_tableau(tableau_in_registers, n_vars, n, placement,
+                                          max_registers))
         self.n, self.n_vars, self.n_params = n, n_vars, p_t.shape[1]
         self.built = _kernels.rodas5p_kernel(ode_fn, n_vars, self.n_params, options)
         # static, structure-of-arrays inputs the graph reads

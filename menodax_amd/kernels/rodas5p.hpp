@@ -93,6 +93,13 @@ __device__ constexpr double H3[8] = {11.419903575922262, 2.8879645146136994, 72.
 #ifndef MDX_ROLLED
 #define MDX_ROLLED 0
 #endif
+// Hold the tableau in vector registers across the step loop (unrolled step).
+// A double that is not an inline constant costs two scalar moves at every
+// use, and when a SIMD runs a single wave every instruction takes an issue
+// slot; loaded once into registers, the step issues none of them.
+#ifndef MDX_TAB_REGS
+#define MDX_TAB_REGS 0
+#endif
 #ifndef MDX_ADJOINT
 #define MDX_ADJOINT 0
 #endif
@@ -535,6 +542,42 @@ __device__ __forceinline__ void sens_stage(int S, const Z& ybase, double t, cons
 #endif
 }
 
+// The tableau as the unrolled step reads it: compile-time constants, or
+// (MDX_TAB_REGS) copies made once per kernel in registers the compiler cannot
+// fold back into constants. Only the entries a step reads are kept live.
+struct Tab {
+#if MDX_TAB_REGS
+  double cm[8][7], am[5][5], dv[8], ct[4];
+  static __device__ __forceinline__ double reg(double x) {
+    asm("" : "+v"(x));
+    return x;
+  }
+  __device__ __forceinline__ Tab() {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+#pragma unroll
+      for (int q = 0; q < 7; ++q) cm[r][q] = reg(CM[r][q]);
+      dv[r] = reg(DV[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) am[r][q] = reg(AM[r][q]);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ct[r] = reg(CT[r]);
+  }
+  __device__ __forceinline__ double c(int r, int q) const { return cm[r][q]; }
+  __device__ __forceinline__ double a(int r, int q) const { return am[r][q]; }
+  __device__ __forceinline__ double d(int r) const { return dv[r]; }
+  __device__ __forceinline__ double t(int r) const { return ct[r]; }
+#else
+  __device__ __forceinline__ double c(int r, int q) const { return CM[r][q]; }
+  __device__ __forceinline__ double a(int r, int q) const { return AM[r][q]; }
+  __device__ __forceinline__ double d(int r) const { return DV[r]; }
+  __device__ __forceinline__ double t(int r) const { return CT[r]; }
+#endif
+};
+
 // Stage S: rhs_j = LU(work_j + dt*d_S*dT_j + (sum_q C_Sq k_qj) / dt), solved
 // against the step's factorisation into k_S (rodas5P.py:611-647). Rows past N
 // (sensitivity blocks) keep their value in double for sens_stage.
@@ -542,19 +585,20 @@ template <int S>
 __device__ __forceinline__ void stage(const double* __restrict__ work,
                                       const double* __restrict__ dT, LUT (*__restrict__ k)[NA],
                                       const LUT* __restrict__ lu, const Pivots& ipiv,
-                                      double dt_use, double inv_dt, double* __restrict__ sv) {
+                                      double dt_use, double inv_dt, double* __restrict__ sv,
+                                      const Tab& tb) {
   LUT r[N];
 #pragma unroll
   for (int j = 0; j < NA; ++j) {
     double v;
     if constexpr (S == 0) {
-      v = fmaT(dt_use * DV[0], DTJ(j), work[j]);
+      v = fmaT(dt_use * tb.d(0), DTJ(j), work[j]);
     } else {
-      double acc = CM[S][0] * (double)k[0][j];
+      double acc = tb.c(S, 0) * (double)k[0][j];
 #pragma unroll
-      for (int q = 1; q < S; ++q) acc = fmaT(CM[S][q], (double)k[q][j], acc);
+      for (int q = 1; q < S; ++q) acc = fmaT(tb.c(S, q), (double)k[q][j], acc);
       if constexpr (S < 5)
-        v = fmaT(acc, inv_dt, fmaT(dt_use * DV[S], DTJ(j), work[j]));
+        v = fmaT(acc, inv_dt, fmaT(dt_use * tb.d(S), DTJ(j), work[j]));
       else
         v = fmaT(acc, inv_dt, work[j]);
     }
@@ -571,12 +615,12 @@ __device__ __forceinline__ void stage(const double* __restrict__ work,
 // u = y + sum_q A_{S+1,q} k_q (rodas5P.py:649-661).
 template <int S>
 __device__ __forceinline__ void advance(double* __restrict__ u, const double* __restrict__ y,
-                                        LUT (*__restrict__ k)[NA]) {
+                                        LUT (*__restrict__ k)[NA], const Tab& tb) {
 #pragma unroll
   for (int j = 0; j < NA; ++j) {
-    double acc = AM[S][0] * (double)k[0][j];
+    double acc = tb.a(S, 0) * (double)k[0][j];
 #pragma unroll
-    for (int q = 1; q <= S; ++q) acc = fmaT(AM[S][q], (double)k[q][j], acc);
+    for (int q = 1; q <= S; ++q) acc = fmaT(tb.a(S, q), (double)k[q][j], acc);
     u[j] = y[j] + acc;
   }
 }
@@ -655,7 +699,8 @@ __device__ __forceinline__ void chunk_sum(const double* __restrict__ c, int S, i
 }
 
 // One attempted step, accepted or rejected, with the saves it passes.
-__device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i) {
+__device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
+                                     const Tab& tb) {
 #if MDX_ROLLED && !MDX_PARAMS_IN_REGS && MDX_NP > 0
   if (a.max_steps < 0) p.v[a.n_save % MDX_NP] = 0;  // keeps the row in memory (see below)
 #endif
@@ -829,37 +874,37 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
 #else
   double work[NA];
   rhs_aug(s.y, t, p, work);
-  stage<0>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  stage<0>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
   sens_stage(0, ybase, t, p, sv, lu, ipiv, ks);
-  advance<0>(u, s.y, k);
-  rhs_aug(u, t + CT[0] * dt_use, p, work);
-  stage<1>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  advance<0>(u, s.y, k, tb);
+  rhs_aug(u, t + tb.t(0) * dt_use, p, work);
+  stage<1>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
   sens_stage(1, ybase, t, p, sv, lu, ipiv, ks);
-  advance<1>(u, s.y, k);
-  rhs_aug(u, t + CT[1] * dt_use, p, work);
-  stage<2>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  advance<1>(u, s.y, k, tb);
+  rhs_aug(u, t + tb.t(1) * dt_use, p, work);
+  stage<2>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
   sens_stage(2, ybase, t, p, sv, lu, ipiv, ks);
-  advance<2>(u, s.y, k);
-  rhs_aug(u, t + CT[2] * dt_use, p, work);
-  stage<3>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  advance<2>(u, s.y, k, tb);
+  rhs_aug(u, t + tb.t(2) * dt_use, p, work);
+  stage<3>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
   sens_stage(3, ybase, t, p, sv, lu, ipiv, ks);
-  advance<3>(u, s.y, k);
-  rhs_aug(u, t + CT[3] * dt_use, p, work);
-  stage<4>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  advance<3>(u, s.y, k, tb);
+  rhs_aug(u, t + tb.t(3) * dt_use, p, work);
+  stage<4>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
   sens_stage(4, ybase, t, p, sv, lu, ipiv, ks);
-  advance<4>(u, s.y, k);
+  advance<4>(u, s.y, k, tb);
   rhs_aug(u, t_end, p, work);
-  stage<5>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  stage<5>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
   sens_stage(5, ybase, t, p, sv, lu, ipiv, ks);
 #pragma unroll
   for (int j = 0; j < NA; ++j) u[j] = u[j] + (double)k[5][j];
   rhs_aug(u, t_end, p, work);
-  stage<6>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  stage<6>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
   sens_stage(6, ybase, t, p, sv, lu, ipiv, ks);
 #pragma unroll
   for (int j = 0; j < NA; ++j) u[j] = u[j] + (double)k[6][j];
   rhs_aug(u, t_end, p, work);
-  stage<7>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv);
+  stage<7>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
   sens_stage(7, ybase, t, p, sv, lu, ipiv, ks);
 #endif
 
@@ -1014,7 +1059,8 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p(
   p.load(a, i);
   State s;
   init(s, a, i, p);
-  while (active(s, a)) step(s, p, a, i);
+  const Tab tb;
+  while (active(s, a)) step(s, p, a, i, tb);
   finish(s, p, a, i);
 }
 
@@ -1029,6 +1075,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p_
   using namespace mdx::r5;
   PA p;
   State s;
+  const Tab tb;
   mdx::i64 i = -1;
   bool exhausted = false;
   for (;;) {
@@ -1050,7 +1097,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p_
     }
     if (!__any(i >= 0)) break;
     if (i >= 0) {
-      if (active(s, a)) step(s, p, a, i);
+      if (active(s, a)) step(s, p, a, i, tb);
       if (!active(s, a)) {
         finish(s, p, a, i);
         i = -1;

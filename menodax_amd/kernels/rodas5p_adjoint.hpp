@@ -253,6 +253,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_rodas5p_adjoint(
   p.load(a, i);
   State s;
   init(s, a, i, p);
+  const Tab tb;
   const int C = g.ckpt_every;
 
   // phase A: the forward solve, checkpointing before every C-th accepted step
@@ -264,7 +265,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_rodas5p_adjoint(
       ++n_ck;
       last = s.accepted;
     }
-    step(s, p, a, i);
+    step(s, p, a, i, tb);
   }
   if (status != 0) n_ck = 0;  // not enough room: report, leave the gradients NaN
 
@@ -289,7 +290,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_rodas5p_adjoint(
       wsat(g, i, b + N + 1) = step_size(s);
       wsat(g, i, b + N + 2) = (double)s.save_idx;
       const int acc0 = s.accepted;
-      step(s, p, a, i);
+      step(s, p, a, i, tb);
       if (s.accepted > acc0) {
         wsat(g, i, b + N + 3) = (double)s.save_idx;
         ++n_rec;

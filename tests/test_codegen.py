@@ -206,6 +206,27 @@ def test_placement_fills_lds_in_order_of_traffic():
         P(50, "fp32", 64, "shared")
 
 
+def test_the_tableau_goes_to_registers_for_small_systems_and_ensembles():
+    """The size rule: the unrolled step of up to three variables, when the
+    ensemble fits two waves per SIMD (256 CUs: 131,072 trajectories), never
+    under a register cap or another placement; only the plain forward kernel
+    gets it, and the kernel still compiles."""
+    T = _kernels.tableau_in_registers
+    assert T(3, 65536, 256) and T(3, 131072, 256) and T(2, 1, 256)
+    assert not T(3, 131073, 256) and not T(4, 1000, 256)
+    assert not T(3, 1000, 256, "rolled") and not T(3, 1000, 256, max_registers=128)
+    assert T(3, 1000, 256, "registers")
+    on = _kernels.KernelOptions(tableau_in_registers=True)
+    src, _ = _kernels.rodas5p_source(S.robertson, 3, 3, on)
+    assert "#define MDX_TAB_REGS 1" in src
+    assert _runtime.rtc_compile(src, "robertson_tab.hip")[:4] == b"\x7fELF"
+    adj, _ = _kernels.rodas5p_source(S.robertson, 3, 3, on, adjoint=True)
+    assert "#define MDX_TAB_REGS 0" in adj
+    rolled, _ = _kernels.rodas5p_source(
+        S.robertson, 3, 3, _kernels.KernelOptions(tableau_in_registers=True, placement="rolled"))
+    assert "#define MDX_TAB_REGS 0" in rolled
+
+
 def test_structurally_zero_time_derivative_is_flagged():
     g = cg.generate(cg.trace(S.robertson, 3, 3), colour=[0, 1, 2], n_colours=3,
                     store=[(r, c, r * 3 + c) for r in range(3) for c in range(3)])

@@ -37,3 +37,16 @@ def test_a_sample_is_timed_in_the_launch_shape_of_the_whole_ensemble():
             lu_precision="fp32", sparsity=normalize_sparsity(mask, 24), placement=c.placement,
             trajectories_per_block=c.trajectories_per_block))
         assert c.refill == rodas5P._use_refill(built, 4096)
+
+
+def test_a_small_system_is_timed_with_and_without_the_tableau_in_registers():
+    y0, p = S.robertson_lognormal_ensemble(4096, seed=3)
+    t = S.robertson_benchmark_times()
+    kw = dict(rtol=1e-6, atol=1e-8, lu_precision="fp32")
+    best, table = autotune.tune(S.robertson, y0, t, p, blocks=(64,), **kw)
+    regs = {c.tableau_in_registers for c in table if c.placement == "registers"}
+    assert regs == {False, True}
+    assert not any(c.tableau_in_registers for c in table if c.placement != "registers")
+    opts = autotune.tuned_options(S.robertson, y0, t, p, blocks=(64,), **kw)
+    np.testing.assert_array_equal(rodas5P.solve(S.robertson, y0, t, p, **opts, **kw),
+                                  rodas5P.solve(S.robertson, y0, t, p, **kw))

@@ -268,6 +268,34 @@ def test_every_placement_is_bit_exact(placement, lu):
         _same(hist, st, ref, rst)
 
 
+@pytest.mark.parametrize("lu", ["fp32", "fp64"])
+def test_the_tableau_in_registers_changes_no_bit(lu):
+    """The unrolled step with its coefficients copied into registers once per
+    kernel, against the oracle, in both launch shapes and for a 2-variable
+    system too; the size rule picks it for these ensembles."""
+    from menodax_amd import _kernels
+
+    y0, p = S.robertson_lognormal_ensemble(3000, seed=2)
+    t = S.robertson_benchmark_times()
+    ref, rst = O.rodas5p(O.robertson(), y0, t, p, rtol=1e-6, atol=1e-8, lu_precision=lu)
+    assert _kernels.tableau_in_registers(3, len(y0), 256)
+    for tab in (None, True, False):
+        for refill in (False, True):
+            hist, st = rodas5P.solve(S.robertson, y0, t, p, rtol=1e-6, atol=1e-8,
+                                     lu_precision=lu, return_stats=True, refill=refill,
+                                     tableau_in_registers=tab)
+            _same(hist, st, ref, rst)
+    fn, _ = S.vdp_lattice(1, mu=100.0, d=0.1, wrap=False)
+    y0v = np.random.default_rng(5).uniform(-2.0, 2.0, (500, 2))
+    tv = np.linspace(0.0, 200.0, 11)
+    refv, rstv = O.rodas5p(O.vdp_lattice(1, mu=100.0, d=0.1, wrap=False), y0v, tv, np.ones((500, 1)), rtol=1e-6,
+                           atol=1e-8, lu_precision=lu)
+    for tab in (True, False):
+        hist, st = rodas5P.solve(fn, y0v, tv, np.ones((500, 1)), rtol=1e-6, atol=1e-8,
+                                 lu_precision=lu, return_stats=True, tableau_in_registers=tab)
+        _same(hist, st, refv, rstv)
+
+
 def test_a_claim_order_changes_no_bit():
     """The persistent launch handing trajectories out in a given order -- the
     costliest first, shuffled, or ranked by a short pilot -- computes every
