@@ -290,7 +290,8 @@ def lorenz_block(torch, dev, n=1024):
     t = torch.tensor(np.linspace(0.0, 1.0, 100), device=dev)
     y0_t = torch.tensor(np.ascontiguousarray(y0.T), device=dev).t()
     p_t = torch.tensor(np.ascontiguousarray(p.T), device=dev).t()
-    built = _kernels.tsit5_kernel(S.lorenz, 3, 1, _kernels.Tsit5Options())
+    built = _kernels.tsit5_kernel(S.lorenz, 3, 1, _kernels.Tsit5Options(
+        tableau_in_registers=rodas5P._tableau(None, 3, n, None, None)))
     outs = rodas5P.launch(built, y0_t, t, p_t, n, dt0=0.0, rtol=RTOL, atol=ATOL,
                           max_steps=100000, refill=False)
 

@@ -389,6 +389,9 @@ class Tsit5Options:
     sens: object = None
     #: "registers", "rolled", "lds" or None (by size); see ``tsit5_placement``
     placement: str | None = None
+    #: keep the coefficients in vector registers across the step loop (the
+    #: unrolled step only), or None for the size rule
+    tableau_in_registers: bool | None = None
 
 
 def tsit5_source(ode_fn, n_vars: int, n_params: int, options: Tsit5Options, adjoint=False):
@@ -432,6 +435,8 @@ def tsit5_source(ode_fn, n_vars: int, n_params: int, options: Tsit5Options, adjo
         "MDX_LDS_K": place["MDX_LDS_K"],
         "MDX_LDS_U": place["MDX_LDS_U"],
         "MDX_LDS_Y": place["MDX_LDS_Y"],
+        "MDX_TAB_REGS": 1 if (place["MDX_ROLLED"] == 0 and options.tableau_in_registers
+                              and not adjoint and spec is None) else 0,
     }
     adj_src = ""
     if adjoint:

@@ -247,8 +247,14 @@ constexpr bool DENSE_UNROLLED = N <= MDX_DENSE_UNROLL_MAX;
 // row divides by its pivot. With float storage each row multiplies by the
 // pivot's double reciprocal (see Pivots); a quotient in the float subnormal
 // range can sit exactly on a midpoint of that coarser grid with u not a power
-// of two, so when any row's result is nonzero and below FLT_MIN the whole
-// substitution is redone with divisions -- rare, and one branch per solve.
+// of two, so when any row's double product is nonzero and below FLT_MIN the
+// whole substitution is redone with divisions -- rare, and one branch per
+// solve. The test reads the products' exponents: |x| < 2^-126 exactly when
+// frexp's exponent is -126 or less, and zero, inf and NaN give 0, so the rows
+// combine with one integer minimum and one compare. It flags every case in
+// which the rounded float would be subnormal or a nonzero quotient would round
+// to zero, plus the products just below FLT_MIN that round up to it, for which
+// the divisions give the same bits.
 template <bool UNROLLED>
 __device__ __forceinline__ void back_substitute(const double* __restrict__ lu, const Pivots& ipiv,
                                                 double* __restrict__ r) {
@@ -266,7 +272,7 @@ template <bool UNROLLED>
 __device__ __forceinline__ void back_substitute(const float* __restrict__ lu, const Pivots& ipiv,
                                                 float* __restrict__ r) {
   float z[N];
-  bool tiny = false;
+  int e_min = 0x7fffffff;
 #pragma unroll
   for (int i = 0; i < N; ++i) z[i] = r[i];
 #pragma unroll
@@ -274,11 +280,12 @@ __device__ __forceinline__ void back_substitute(const float* __restrict__ lu, co
     float acc = r[i];
 #pragma unroll
     for (int j = i + 1; j < N; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
-    const float q = (float)((double)acc * ipiv.rd[i]);
-    tiny |= __builtin_fabsf(q) < 1.17549435e-38f && acc != 0.0f;
-    r[i] = q;
+    const double qd = (double)acc * ipiv.rd[i];
+    const int e = __builtin_amdgcn_frexp_exp(qd);
+    e_min = e < e_min ? e : e_min;
+    r[i] = (float)qd;
   }
-  if (tiny) {
+  if (e_min <= -126) {
     // the first row's divisor passes through an opaque move so the divisions
     // stay on this branch instead of being computed on every solve
     float u0 = lu[0];

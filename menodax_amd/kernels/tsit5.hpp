@@ -71,6 +71,39 @@ constexpr double E1 = 0.0017800110522257773, E2 = 0.0008164344596567463,
 #ifndef MDX_ADJOINT
 #define MDX_ADJOINT 0
 #endif
+// Hold the tableau in vector registers across the step loop (unrolled step).
+// A double that is not an inline constant costs two scalar moves at every
+// use, and when a SIMD runs a single wave every instruction takes an issue
+// slot; loaded once into registers, the step issues none of them.
+#ifndef MDX_TAB_REGS
+#define MDX_TAB_REGS 0
+#endif
+
+#define MDX_T5_COEFS(X)                                                                       \
+  X(C2) X(C3) X(C4) X(C5) X(A21) X(A31) X(A32) X(A41) X(A42) X(A43) X(A51) X(A52) X(A53)      \
+  X(A54) X(A61) X(A62) X(A63) X(A64) X(A65) X(B1) X(B2) X(B3) X(B4) X(B5) X(B6) X(E1) X(E2) \
+  X(E3) X(E4) X(E5) X(E6) X(E7)
+
+// The coefficients as the unrolled step reads them: compile-time constants,
+// or (MDX_TAB_REGS) copies made once per kernel in registers the compiler
+// cannot fold back into constants.
+struct Tab {
+#if MDX_TAB_REGS
+#define MDX_T5_FIELD(c) double c;
+#define MDX_T5_LOAD(c) c = reg(::mdx::t5::c);
+  MDX_T5_COEFS(MDX_T5_FIELD)
+  static __device__ __forceinline__ double reg(double x) {
+    asm("" : "+v"(x));
+    return x;
+  }
+  __device__ __forceinline__ Tab() { MDX_T5_COEFS(MDX_T5_LOAD) }
+#undef MDX_T5_LOAD
+#else
+#define MDX_T5_FIELD(c) static constexpr double c = ::mdx::t5::c;
+  MDX_T5_COEFS(MDX_T5_FIELD)
+#endif
+#undef MDX_T5_FIELD
+};
 
 // Stage rows 2..7 (the last is the solution weights B) and abscissae, for the
 // rolled step and the adjoint.
@@ -193,7 +226,8 @@ __device__ __forceinline__ bool active(const State& s, const SolveArgs& a) {
 // term a product, then one fma per stage vector, then the fma with dt and y),
 // so the bits are the same. For scratch placement, the never-executed stores
 // (max_steps >= 0 is checked before launch) keep the arrays in memory.
-__device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i) {
+__device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
+                                     const Tab& tb) {
   double dt_use = s.dt;
   if (dt_use > s.tf - s.t) dt_use = s.tf - s.t;
   if (dt_use < 1e-30) dt_use = 1e-30;
@@ -319,7 +353,8 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
   s.n_steps++;
 }
 #else
-__device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i) {
+__device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
+                                     const Tab& tb) {
   double dt_use = s.dt;
   if (dt_use > s.tf - s.t) dt_use = s.tf - s.t;
   if (dt_use < 1e-30) dt_use = 1e-30;
@@ -332,46 +367,46 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
     sys::rhs(s.y, t, p, k1);
   }
 #pragma unroll
-  for (int j = 0; j < N; ++j) u[j] = fmaT(dt_use, A21 * k1[j], s.y[j]);
-  sys::rhs(u, t + C2 * dt_use, p, k2);
+  for (int j = 0; j < N; ++j) u[j] = fmaT(dt_use, tb.A21 * k1[j], s.y[j]);
+  sys::rhs(u, t + tb.C2 * dt_use, p, k2);
 #pragma unroll
-  for (int j = 0; j < N; ++j) u[j] = fmaT(dt_use, fmaT(A32, k2[j], A31 * k1[j]), s.y[j]);
-  sys::rhs(u, t + C3 * dt_use, p, k3);
+  for (int j = 0; j < N; ++j) u[j] = fmaT(dt_use, fmaT(tb.A32, k2[j], tb.A31 * k1[j]), s.y[j]);
+  sys::rhs(u, t + tb.C3 * dt_use, p, k3);
 #pragma unroll
   for (int j = 0; j < N; ++j) {
-    double q = A41 * k1[j];
-    q = fmaT(A42, k2[j], q);
-    q = fmaT(A43, k3[j], q);
+    double q = tb.A41 * k1[j];
+    q = fmaT(tb.A42, k2[j], q);
+    q = fmaT(tb.A43, k3[j], q);
     u[j] = fmaT(dt_use, q, s.y[j]);
   }
-  sys::rhs(u, t + C4 * dt_use, p, k4);
+  sys::rhs(u, t + tb.C4 * dt_use, p, k4);
 #pragma unroll
   for (int j = 0; j < N; ++j) {
-    double q = A51 * k1[j];
-    q = fmaT(A52, k2[j], q);
-    q = fmaT(A53, k3[j], q);
-    q = fmaT(A54, k4[j], q);
+    double q = tb.A51 * k1[j];
+    q = fmaT(tb.A52, k2[j], q);
+    q = fmaT(tb.A53, k3[j], q);
+    q = fmaT(tb.A54, k4[j], q);
     u[j] = fmaT(dt_use, q, s.y[j]);
   }
-  sys::rhs(u, t + C5 * dt_use, p, k5);
+  sys::rhs(u, t + tb.C5 * dt_use, p, k5);
 #pragma unroll
   for (int j = 0; j < N; ++j) {
-    double q = A61 * k1[j];
-    q = fmaT(A62, k2[j], q);
-    q = fmaT(A63, k3[j], q);
-    q = fmaT(A64, k4[j], q);
-    q = fmaT(A65, k5[j], q);
+    double q = tb.A61 * k1[j];
+    q = fmaT(tb.A62, k2[j], q);
+    q = fmaT(tb.A63, k3[j], q);
+    q = fmaT(tb.A64, k4[j], q);
+    q = fmaT(tb.A65, k5[j], q);
     u[j] = fmaT(dt_use, q, s.y[j]);
   }
   sys::rhs(u, t + C6 * dt_use, p, k6);
 #pragma unroll
   for (int j = 0; j < N; ++j) {
-    double q = B1 * k1[j];
-    q = fmaT(B2, k2[j], q);
-    q = fmaT(B3, k3[j], q);
-    q = fmaT(B4, k4[j], q);
-    q = fmaT(B5, k5[j], q);
-    q = fmaT(B6, k6[j], q);
+    double q = tb.B1 * k1[j];
+    q = fmaT(tb.B2, k2[j], q);
+    q = fmaT(tb.B3, k3[j], q);
+    q = fmaT(tb.B4, k4[j], q);
+    q = fmaT(tb.B5, k5[j], q);
+    q = fmaT(tb.B6, k6[j], q);
     u[j] = fmaT(dt_use, q, s.y[j]);
   }
   sys::rhs(u, t + C7 * dt_use, p, s.k7);
@@ -379,13 +414,13 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i)
   double err = 0.0;
 #pragma unroll
   for (int j = 0; j < N; ++j) {
-    double e = E1 * k1[j];
-    e = fmaT(E2, k2[j], e);
-    e = fmaT(E3, k3[j], e);
-    e = fmaT(E4, k4[j], e);
-    e = fmaT(E5, k5[j], e);
-    e = fmaT(E6, k6[j], e);
-    e = fmaT(E7, s.k7[j], e);
+    double e = tb.E1 * k1[j];
+    e = fmaT(tb.E2, k2[j], e);
+    e = fmaT(tb.E3, k3[j], e);
+    e = fmaT(tb.E4, k4[j], e);
+    e = fmaT(tb.E5, k5[j], e);
+    e = fmaT(tb.E6, k6[j], e);
+    e = fmaT(tb.E7, s.k7[j], e);
     const double err_est = dt_use * e;
     const double ya = absT(s.y[j]), yb = absT(u[j]);
     const double scale = fmaT(a.rtol, ya > yb ? ya : yb, a.atol);
@@ -473,7 +508,8 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_tsit5(mdx::SolveArgs
   p.load(a, i);
   State s;
   init(s, a, i);
-  while (active(s, a)) step(s, p, a, i);
+  const Tab tb;
+  while (active(s, a)) step(s, p, a, i, tb);
   finish(s, a, i);
 }
 
@@ -481,6 +517,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_tsit5_refill(mdx::So
   using namespace mdx::t5;
   PA p;
   State s;
+  const Tab tb;
   mdx::i64 i = -1;
   bool exhausted = false;
   for (;;) {
@@ -502,7 +539,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_tsit5_refill(mdx::So
     }
     if (!__any(i >= 0)) break;
     if (i >= 0) {
-      if (active(s, a)) step(s, p, a, i);
+      if (active(s, a)) step(s, p, a, i, tb);
       if (!active(s, a)) {
         finish(s, a, i);
         i = -1;

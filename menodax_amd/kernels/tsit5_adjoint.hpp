@@ -169,6 +169,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_tsit5_adjoint(mdx::A
   p.load(a, i);
   State s;
   init(s, a, i);
+  const Tab tb;
   const int C = g.ckpt_every;
 
   // phase A: the forward solve, checkpointing before every C-th accepted step
@@ -180,7 +181,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_tsit5_adjoint(mdx::A
       ++n_ck;
       last = s.accepted;
     }
-    step(s, p, a, i);
+    step(s, p, a, i, tb);
   }
   if (status != 0) n_ck = 0;  // not enough room: report, leave the gradients NaN
 
@@ -204,7 +205,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) void mdx_tsit5_adjoint(mdx::A
       wsat(g, i, b + N + 1) = step_size(s);
       wsat(g, i, b + N + 2) = (double)s.save_idx;
       const int acc0 = s.accepted;
-      step(s, p, a, i);
+      step(s, p, a, i, tb);
       if (s.accepted > acc0) {
         wsat(g, i, b + N + 3) = (double)s.save_idx;
         ++n_rec;

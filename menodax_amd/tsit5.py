@@ -12,6 +12,9 @@ Differentiation and ``torch.func.vmap`` work as for ``rodas5P.solve``:
 gradients come from the forward-sensitivity joint solve by default, or, with
 ``adjoint=True``, from the discrete-adjoint kernel (checkpointed every
 ``ckpt_every`` accepted steps), which ``adjoint_gradients`` also runs directly.
+``tableau_in_registers`` holds the method's coefficients in registers for the
+unrolled step (``None`` = by system and ensemble size, as for Rodas5P); the
+bits are the same either way.
 """
 
 from __future__ import annotations
@@ -20,7 +23,7 @@ from dataclasses import replace
 
 from . import _autodiff, _common, _kernels
 from ._kernels import Tsit5Options
-from .rodas5P import finish, launch
+from .rodas5P import _tableau, finish, launch
 
 _SHARED_MAX_NVARS = 16
 
@@ -62,6 +65,7 @@ def solve(
     work_order=None,
     adjoint=False,
     ckpt_every=64,
+    tableau_in_registers=None,
 ):
     """Integrate an ensemble with Tsit5; see the module docstring."""
     if _autodiff.under_vmap(y0, t_span, params):
@@ -69,7 +73,8 @@ def solve(
         kwargs = dict(rtol=rtol, atol=atol, first_step=first_step, max_steps=max_steps,
                       return_stats=True, error_weights=error_weights, pcoeff=pcoeff,
                       icoeff=icoeff, dcoeff=dcoeff, backend=backend,
-                      trajectories_per_block=trajectories_per_block, refill=refill, stream=stream)
+                      trajectories_per_block=trajectories_per_block, refill=refill, stream=stream,
+                      tableau_in_registers=tableau_in_registers)
 
         def run(y, t, p):
             hist, stats = solve(ode_fn, y, t, p, **kwargs)
@@ -90,6 +95,7 @@ def solve(
         dcoeff=float(dcoeff),
         trajectories_per_block=_kernels.trajectories_per_block_or_default(trajectories_per_block),
         unit_weights=error_weights is None,
+        tableau_in_registers=_tableau(tableau_in_registers, n_vars, n, None, None),
     )
     times = _common.as_f64_device(t_span, dev).contiguous()
     if times.ndim != 1 or times.numel() < 1:

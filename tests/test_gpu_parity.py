@@ -296,6 +296,24 @@ def test_the_tableau_in_registers_changes_no_bit(lu):
         _same(hist, st, refv, rstv)
 
 
+@pytest.mark.parametrize("tab", [False, True])
+def test_subnormal_stage_quotients_match_the_divisions(tab):
+    """States of 1e-41..1e-34 put the FP32 back substitution's quotients in
+    the float subnormal range, and some below half the smallest subnormal:
+    the solves that produce them are redone with divisions, so every bit
+    still matches the oracle, which always divides."""
+    fn, _, (rows, cols, vals, pidx) = S.arrow_system(3)
+    n_p = S.arrow_n_params(3)
+    y0, lam = S.arrow_ensemble(512, n=3)
+    y0 = y0 * 10.0 ** np.random.default_rng(7).uniform(-41.0, -34.0, (512, 1))
+    t = np.linspace(0.0, 2.0, 5)
+    system = O.linquad(3, rows, cols, vals, 0.01, off_pidx=pidx, n_p=n_p)
+    ref, rst = O.rodas5p(system, y0, t, lam, rtol=1e-6, atol=1e-8, lu_precision="fp32")
+    hist, st = rodas5P.solve(fn, y0, t, lam, rtol=1e-6, atol=1e-8, lu_precision="fp32",
+                             return_stats=True, tableau_in_registers=tab)
+    _same(hist, st, ref, rst)
+
+
 def test_a_claim_order_changes_no_bit():
     """The persistent launch handing trajectories out in a given order -- the
     costliest first, shuffled, or ranked by a short pilot -- computes every
@@ -337,13 +355,16 @@ def test_array_rhs_stage_evaluations_are_bit_identical():
 
 
 # --- Tsit5 --------------------------------------------------------------------
+@pytest.mark.parametrize("tab", [None, False, True])
 @pytest.mark.parametrize("refill", [False, True])
-def test_lorenz_rho_ensemble_bit_exact(refill):
-    """The Tsit5 configuration: rho ~ U[0, 28], 1,024 trajectories, 100 saves on [0, 1]."""
+def test_lorenz_rho_ensemble_bit_exact(refill, tab):
+    """The Tsit5 configuration: rho ~ U[0, 28], 1,024 trajectories, 100 saves on
+    [0, 1]; with the coefficients as constants or held in registers (the size
+    rule's choice here)."""
     y0, p = S.lorenz_rho_ensemble(1024)
     t = np.linspace(0.0, 1.0, 100)
     hist, st = tsit5.solve(S.lorenz, y0, t, p, rtol=1e-6, atol=1e-8, return_stats=True,
-                           refill=refill)
+                           refill=refill, tableau_in_registers=tab)
     ref, rst = O.tsit5(O.lorenz(), y0, t, p, rtol=1e-6, atol=1e-8)
     _same(hist, st, ref, rst)
 
