@@ -36,6 +36,28 @@ def eager_only(fn):
         return fn
 
 
+def compilable(eager, traced):
+    """``eager`` as the public entry: it runs eagerly, except while
+    ``torch.compile`` traces a call that ``traced`` expresses with the custom
+    operators of ``_ops`` (``traced`` returns NotImplemented otherwise, and the
+    compiler stops its graph at the eager call: a graph break)."""
+    import functools
+
+    import torch
+
+    disabled = eager_only(eager)
+
+    @functools.wraps(eager)
+    def entry(*args, **kwargs):
+        if torch.compiler.is_compiling():
+            out = traced(*args, **kwargs)
+            if out is not NotImplemented:
+                return out
+        return disabled(*args, **kwargs)
+
+    return entry
+
+
 def device():
     torch = _torch()
     if not torch.cuda.is_available():

@@ -221,7 +221,30 @@ def solve_sensitivities(ode_fn, y0, t_span, params, *, wrt_y0=True, wrt_params=T
     return (state, sens, stats) if return_stats else (state, sens)
 
 
-# Called from torch.compile'd code these run eagerly (see _common.eager_only).
-solve = _common.eager_only(solve)
+def _traced(ode_fn, y0, t_span, params, *, rtol=1e-8, atol=1e-10, first_step=None,
+            max_steps=100000, return_stats=False, error_weights=None, pcoeff=0.0, icoeff=1.0,
+            dcoeff=0.0, backend="auto", sens_error_control=True, sens_param_columns=None,
+            trajectories_per_block=None, refill=None, stream=None, work_order=None,
+            adjoint=False, ckpt_every=64, tableau_in_registers=None):
+    """``solve`` while ``torch.compile`` traces it (see rodas5P._traced)."""
+    from . import _ops
+
+    if error_weights is not None or stream is not None or work_order is not None:
+        return NotImplemented
+    if hasattr(y0, "shape") and len(y0.shape) in (1, 2):
+        _check_backend(y0.shape[-1], backend)
+    options = (("pcoeff", float(pcoeff)), ("icoeff", float(icoeff)), ("dcoeff", float(dcoeff)),
+               ("trajectories_per_block",
+                _kernels.trajectories_per_block_or_default(trajectories_per_block)),
+               ("unit_weights", True))
+    return _ops.traced_solve("tsit5", ode_fn, y0, t_span, params, options=options,
+                             sparsity=None, tableau=tableau_in_registers, rtol=rtol, atol=atol,
+                             first_step=first_step, max_steps=max_steps, refill=refill,
+                             adjoint=adjoint, ckpt_every=ckpt_every, return_stats=return_stats)
+
+
+# Traced by torch.compile, a plain solve is one custom operator (menodax::
+# ensemble_solve); everything else runs eagerly (see _common.eager_only).
+solve = _common.compilable(solve, _traced)
 adjoint_gradients = _common.eager_only(adjoint_gradients)
 solve_sensitivities = _common.eager_only(solve_sensitivities)
