@@ -581,7 +581,7 @@ def reference_benchmarks(torch, log_fn, reps=5):
             if name.startswith("dense"):
                 # the autotuner's placement, workgroup size and launch shape
                 # (tuning, like compilation, outside the timed calls)
-                opts = autotune.tuned_options(fn, y0, t_v, p, reps=1, blocks=(16, 32, 64),
+                opts = autotune.tuned_options(fn, y0, t_v, p, reps=1, blocks=(4, 8, 16, 32, 64),
                                               **kw, **extra)
                 tuned = {"default_ms": ms, "tuned": opts}
                 ms = clock(lambda: rodas5P.solve(fn, y0, t_v, p, **kw, **extra, **opts))

@@ -1128,10 +1128,11 @@ def generate_neg_jac(traced: Traced, *, colour, n_colours, store):
     dT_lines = [f"  dT[{j}] = {value_of(n)};" for j, n in enumerate(dt_tan)]
     src = "\n".join(
         [
-            "template <class PA, class LUT, class YA>",
+            "template <class PA, class LM, class YA>",
             "__device__ __forceinline__ void neg_jac(const YA& y, const double t,",
-            "                                        const PA& p, LUT* __restrict__ lu,",
+            "                                        const PA& p, LM lu,",
             "                                        double* __restrict__ dT) {",
+            "  typedef typename ::mdx::lu_elem<LM>::type LUT;",
             "  (void)t;",
             "  (void)p;",
         ]

@@ -22,6 +22,9 @@ DEFAULT_BLOCK = 64
 PARAMS_IN_REGS_MAX = 16
 #: systems up to this size run the fully unrolled, register-resident step
 UNROLLED_MAX_VARS = 16
+#: the dense LU is unrolled into registers up to this size (MDX_DENSE_UNROLL_MAX);
+#: above it the matrix lives in per-lane scratch or LDS
+DENSE_UNROLL_MAX = 16
 #: the adjoint kernel holds eight stage vectors, their adjoints and the
 #: factorisation in both precisions; unrolled only up to this many variables
 ADJOINT_UNROLLED_MAX_VARS = 8
@@ -107,28 +110,32 @@ def tableau_in_registers(n_vars: int, n_traj: int, n_cu: int, placement=None,
     return n_traj <= n_cu * 4 * 2 * 64
 
 
-def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None) -> dict:
+def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None,
+                      dense_matrix: int = 0) -> dict:
     """The placement macros for a Rodas5P kernel.
 
     Up to ``UNROLLED_MAX_VARS`` the whole step stays in registers. Above it,
-    the stage loop is rolled and the per-lane vectors go to LDS, laid out
-    ``[element][lane]``, in order of traffic per step: the eight stage vectors
-    (about 50 element reads per component per step), the stage argument and
-    the state; whatever does not fit in one compute unit's LDS next to them
-    stays in scratch. Outputs are bit-identical across placements.
+    the stage loop is rolled and the per-lane arrays go to LDS, laid out
+    ``[element][lane]``, in order of traffic per step: a dense iteration
+    matrix of ``dense_matrix`` elements (its factorisation touches each
+    element about n/3 times a step), the eight stage vectors (about 50 element
+    reads per component per step), the stage argument and the state; whatever
+    does not fit in one compute unit's LDS next to them stays in scratch.
+    Outputs are bit-identical across placements.
     """
     if placement is None:
         placement = "registers" if n_vars <= UNROLLED_MAX_VARS else "lds"
     if placement not in PLACEMENTS:
         raise ValueError(f"placement must be one of {PLACEMENTS}, got {placement!r}")
     defs = {"MDX_ROLLED": 0 if placement == "registers" else 1,
-            "MDX_LDS_K": 0, "MDX_LDS_U": 0, "MDX_LDS_Y": 0}
+            "MDX_LDS_LU": 0, "MDX_LDS_K": 0, "MDX_LDS_U": 0, "MDX_LDS_Y": 0}
     lds = 0
     if placement == "lds":
         lu_bytes = 4 if lu_precision == "fp32" else 8
-        for macro, per_lane in (("MDX_LDS_K", 8 * n_vars * lu_bytes),
+        for macro, per_lane in (("MDX_LDS_LU", dense_matrix * lu_bytes),
+                                ("MDX_LDS_K", 8 * n_vars * lu_bytes),
                                 ("MDX_LDS_U", 8 * n_vars), ("MDX_LDS_Y", 8 * n_vars)):
-            if lds + per_lane * block <= LDS_BYTES_PER_CU:
+            if per_lane and lds + per_lane * block <= LDS_BYTES_PER_CU:
                 defs[macro] = 1
                 lds += per_lane * block
     defs["lds_bytes"] = lds
@@ -287,7 +294,11 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
             placement = "registers" if n_aug <= ADJOINT_UNROLLED_MAX_VARS else "rolled"
         elif placement == "lds":
             placement = "rolled"
-    place = rodas5p_placement(n_aug, options.lu_precision, block, placement)
+    # a dense matrix above the unrolled size lives in per-lane memory: LDS
+    # when it fits (the forward step only)
+    dense_matrix = (structure.size if (options.sparsity is None and n_vars > DENSE_UNROLL_MAX
+                                       and not adjoint) else 0)
+    place = rodas5p_placement(n_aug, options.lu_precision, block, placement, dense_matrix)
     defs = {
         "MDX_N": n_vars,
         "MDX_NP": n_params,
@@ -306,6 +317,7 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
         "MDX_DT_ZERO": 1 if gen.dt_zero else 0,
         "MDX_NSENS": 0 if spec is None else spec.n_sens,
         "MDX_ROLLED": place["MDX_ROLLED"],
+        "MDX_LDS_LU": place["MDX_LDS_LU"],
         "MDX_LDS_K": place["MDX_LDS_K"],
         "MDX_LDS_U": place["MDX_LDS_U"],
         "MDX_LDS_Y": place["MDX_LDS_Y"],

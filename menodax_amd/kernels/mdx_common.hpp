@@ -12,6 +12,17 @@
 
 namespace mdx {
 
+// Element type of an LU buffer: a pointer, or an accessor that names it.
+template <class L>
+struct lu_elem {
+  typedef typename L::value_type type;
+};
+template <class T>
+struct lu_elem<T*> {
+  typedef T type;
+};
+
+
 typedef long long i64;
 typedef unsigned long long u64;
 

@@ -136,8 +136,18 @@ typedef GlobalParams PA;
 #ifndef MDX_CHUNK
 #define MDX_CHUNK 16
 #endif
-#if (MDX_LDS_K || MDX_LDS_Y || MDX_LDS_U) && !MDX_ROLLED
+// Columns per chunk in the scratch-resident dense factorisation's row update.
+#ifndef MDX_LU_CHUNK
+#define MDX_LU_CHUNK MDX_CHUNK
+#endif
+#ifndef MDX_LDS_LU
+#define MDX_LDS_LU 0
+#endif
+#if (MDX_LDS_K || MDX_LDS_Y || MDX_LDS_U || MDX_LDS_LU) && !MDX_ROLLED
 #error "LDS placement needs the rolled stage loop"
+#endif
+#if MDX_LDS_LU && (MDX_SPARSE || MDX_ADJOINT)
+#error "the LDS iteration matrix is for the dense forward step"
 #endif
 #ifndef MDX_NSENS
 #define MDX_NSENS 0
@@ -173,7 +183,7 @@ struct Shifted {
 // every LDS address is that one register plus an immediate offset; otherwise
 // the compiler hoists hundreds of per-element addresses out of the step loop
 // and spills them.
-#if MDX_LDS_K || MDX_LDS_Y || MDX_LDS_U
+#if MDX_LDS_K || MDX_LDS_Y || MDX_LDS_U || MDX_LDS_LU
 __device__ __forceinline__ unsigned opaque_lane() {
   unsigned v = threadIdx.x;
   asm volatile("; lane" : "+v"(v));
@@ -200,6 +210,16 @@ __shared__ double lds_u[NA * MDX_BLOCK];
 #define UJ(j) lds_u[(j) * MDX_BLOCK + lane_]
 #else
 #define UJ(j) u[j]
+#endif
+#if MDX_LDS_LU
+// The dense iteration matrix in LDS, element e of this lane at [e][lane]: the
+// factorisation's loads then take LDS latency instead of scratch latency.
+__shared__ LUT lds_lu[MDX_NLU * MDX_BLOCK];
+struct LdsMat {
+  typedef LUT value_type;
+  LUT* base;
+  __device__ __forceinline__ LUT& operator[](int e) const { return base[e * MDX_BLOCK]; }
+};
 #endif
 // A vector in LDS as the generated right-hand side reads it.
 struct LdsVec {
@@ -242,6 +262,7 @@ struct Pivots {
 #define MDX_DENSE_UNROLL_MAX 16
 #endif
 constexpr bool DENSE_UNROLLED = N <= MDX_DENSE_UNROLL_MAX;
+static_assert(!(MDX_LDS_LU && DENSE_UNROLLED), "an unrolled dense matrix stays in registers");
 
 // U x = z, the back substitution of the dense solve. With double storage each
 // row divides by its pivot. With float storage each row multiplies by the
@@ -302,8 +323,13 @@ __device__ __forceinline__ void back_substitute(const float* __restrict__ lu, co
   }
 }
 
-template <class T>
-__device__ __forceinline__ void dense_factor(T* __restrict__ lu, Pivots& ipiv) {
+// An index into a chunk clamped to the last element (a load that stays in
+// bounds; its value is not used).
+__device__ __forceinline__ int clampN(int x) { return x < N ? x : N - 1; }
+
+template <class L>
+__device__ __forceinline__ void dense_factor(L lu, Pivots& ipiv) {
+  typedef typename lu_elem<L>::type T;
   if constexpr (DENSE_UNROLLED) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -343,46 +369,85 @@ __device__ __forceinline__ void dense_factor(T* __restrict__ lu, Pivots& ipiv) {
       }
     }
   } else {
+    // Above the unrolled size the matrix lives in per-lane scratch. Every loop
+    // loads a chunk of MDX_CHUNK elements before computing with it, so that
+    // many loads are in flight per wait; each element still sees the same
+    // operations in the same order. Chunk indices past the end are clamped
+    // to the last element for the loads and skipped for the stores.
 #pragma unroll 1
     for (int i = 0; i < N; ++i) {
       T maxv = absT(lu[i * N + i]);
       int piv = i;
 #pragma unroll 1
-      for (int k = i + 1; k < N; ++k) {
-        const T v = absT(lu[k * N + i]);
-        if (v > maxv) {
-          maxv = v;
-          piv = k;
-        }
+      for (int k0 = i + 1; k0 < N; k0 += MDX_CHUNK) {
+        T v[MDX_CHUNK];
+#pragma unroll
+        for (int c = 0; c < MDX_CHUNK; ++c) v[c] = absT(lu[clampN(k0 + c) * N + i]);
+#pragma unroll
+        for (int c = 0; c < MDX_CHUNK; ++c)
+          if (k0 + c < N && v[c] > maxv) {
+            maxv = v[c];
+            piv = k0 + c;
+          }
       }
       ipiv.p[i] = piv;
       if (piv != i) {
 #pragma unroll 1
-        for (int j = 0; j < N; ++j) {
-          const T a = lu[i * N + j];
-          lu[i * N + j] = lu[piv * N + j];
-          lu[piv * N + j] = a;
+        for (int j0 = 0; j0 < N; j0 += MDX_CHUNK) {
+          T a[MDX_CHUNK], b[MDX_CHUNK];
+#pragma unroll
+          for (int c = 0; c < MDX_CHUNK; ++c) {
+            a[c] = lu[i * N + clampN(j0 + c)];
+            b[c] = lu[piv * N + clampN(j0 + c)];
+          }
+#pragma unroll
+          for (int c = 0; c < MDX_CHUNK; ++c)
+            if (j0 + c < N) {
+              lu[i * N + j0 + c] = b[c];
+              lu[piv * N + j0 + c] = a[c];
+            }
         }
       }
       const T pv = lu[i * N + i];
       const double inv = 1.0 / (double)pv;
       ipiv.rd[i] = inv;
       if (pv != (T)0) {
+        // the pivot row's chunk stays in registers while every row below
+        // takes its update; the multiplier f = a_ki / u_ii (in double) is
+        // formed again for each chunk from a_ki, which is replaced by the
+        // stored L entry only after the last chunk
 #pragma unroll 1
-        for (int k = i + 1; k < N; ++k) {
-          const double f = (double)lu[k * N + i] * inv;
-          lu[k * N + i] = (T)f;
-#pragma unroll 4
-          for (int j = i + 1; j < N; ++j)
-            lu[k * N + j] = (T)fmaT(-f, (double)lu[i * N + j], (double)lu[k * N + j]);
+        for (int j0 = i + 1; j0 < N; j0 += MDX_LU_CHUNK) {
+          double prow[MDX_LU_CHUNK];
+#pragma unroll
+          for (int c = 0; c < MDX_LU_CHUNK; ++c) prow[c] = (double)lu[i * N + clampN(j0 + c)];
+#pragma unroll 1
+          for (int k = i + 1; k < N; ++k) {
+            const double f = (double)lu[k * N + i] * inv;
+            T kv[MDX_LU_CHUNK];
+#pragma unroll
+            for (int c = 0; c < MDX_LU_CHUNK; ++c) kv[c] = lu[k * N + clampN(j0 + c)];
+#pragma unroll
+            for (int c = 0; c < MDX_LU_CHUNK; ++c)
+              if (j0 + c < N) lu[k * N + j0 + c] = (T)fmaT(-f, prow[c], (double)kv[c]);
+          }
+        }
+#pragma unroll 1
+        for (int k0 = i + 1; k0 < N; k0 += MDX_CHUNK) {
+          T l[MDX_CHUNK];
+#pragma unroll
+          for (int c = 0; c < MDX_CHUNK; ++c) l[c] = lu[clampN(k0 + c) * N + i];
+#pragma unroll
+          for (int c = 0; c < MDX_CHUNK; ++c)
+            if (k0 + c < N) lu[(k0 + c) * N + i] = (T)((double)l[c] * inv);
         }
       }
     }
   }
 }
 
-template <class T>
-__device__ __forceinline__ void dense_solve(const T* __restrict__ lu, const Pivots& ipiv,
+template <class L, class T>
+__device__ __forceinline__ void dense_solve(const L& lu, const Pivots& ipiv,
                                             T* __restrict__ r) {
   if constexpr (DENSE_UNROLLED) {
 #pragma unroll
@@ -402,6 +467,8 @@ __device__ __forceinline__ void dense_solve(const T* __restrict__ lu, const Pivo
     }
     back_substitute<true>(lu, ipiv, r);
   } else {
+    // scratch-resident matrix and vector: each row's sum loads a chunk of the
+    // row and of the vector before its multiply-adds, which stay in order
 #pragma unroll 1
     for (int i = 0; i < N; ++i) {
       const int piv = ipiv.p[i];
@@ -411,15 +478,35 @@ __device__ __forceinline__ void dense_solve(const T* __restrict__ lu, const Pivo
         r[piv] = a;
       }
       T acc = r[i];
-#pragma unroll 4
-      for (int j = 0; j < i; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
+#pragma unroll 1
+      for (int j0 = 0; j0 < i; j0 += MDX_CHUNK) {
+        T lv[MDX_CHUNK], rv[MDX_CHUNK];
+#pragma unroll
+        for (int c = 0; c < MDX_CHUNK; ++c) {
+          lv[c] = lu[i * N + clampN(j0 + c)];
+          rv[c] = r[clampN(j0 + c)];
+        }
+#pragma unroll
+        for (int c = 0; c < MDX_CHUNK; ++c)
+          if (j0 + c < i) acc = fmaT(-lv[c], rv[c], acc);
+      }
       r[i] = acc;
     }
 #pragma unroll 1
     for (int i = N - 1; i >= 0; --i) {
       T acc = r[i];
-#pragma unroll 4
-      for (int j = i + 1; j < N; ++j) acc = fmaT(-lu[i * N + j], r[j], acc);
+#pragma unroll 1
+      for (int j0 = i + 1; j0 < N; j0 += MDX_CHUNK) {
+        T uv[MDX_CHUNK], rv[MDX_CHUNK];
+#pragma unroll
+        for (int c = 0; c < MDX_CHUNK; ++c) {
+          uv[c] = lu[i * N + clampN(j0 + c)];
+          rv[c] = r[clampN(j0 + c)];
+        }
+#pragma unroll
+        for (int c = 0; c < MDX_CHUNK; ++c)
+          if (j0 + c < N) acc = fmaT(-uv[c], rv[c], acc);
+      }
       r[i] = acc / lu[i * N + i];
     }
   }
@@ -461,7 +548,8 @@ __device__ __forceinline__ int diag_slot(int d) {
 #endif
 }
 
-__device__ __forceinline__ void factor(LUT* __restrict__ lu, Pivots& ipiv) {
+template <class L>
+__device__ __forceinline__ void factor(L lu, Pivots& ipiv) {
 #if MDX_SPARSE
   (void)ipiv;
   sys::sparse_factor(lu);
@@ -726,7 +814,11 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
 #endif
 
   // M = I/(h gamma) - J straight into the buffer the factorisation works in.
+#if MDX_LDS_LU
+  const LdsMat lu{&lds_lu[lane_]};
+#else
   LUT lu[MDX_NLU];
+#endif
 #if MDX_NEEDS_CLEAR
 #pragma unroll
   for (int q = 0; q < MDX_NLU; ++q) lu[q] = (LUT)0;
