@@ -143,6 +143,21 @@ def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None
     return defs
 
 
+def tsit5_block(requested, n_vars: int) -> int:
+    """The workgroup size of a Tsit5 solve: the caller's, else a wavefront,
+    except that a rolled step (more than ``UNROLLED_MAX_VARS`` variables)
+    takes the largest of 64, 32 and 16 lanes at which its seven stage vectors
+    fit in LDS. The reference's non-stiff Van der Pol rings, 1,000
+    trajectories: 64 variables 0.81 ms at 64 lanes (stage vectors in scratch),
+    0.55 ms at 32; 128 variables 1.60 ms at 64 lanes, 1.05 ms at 16."""
+    if requested is not None or n_vars <= UNROLLED_MAX_VARS:
+        return trajectories_per_block_or_default(requested)
+    for b in (64, 32, 16):
+        if tsit5_placement(n_vars, b)["MDX_LDS_K"]:
+            return b
+    return DEFAULT_BLOCK
+
+
 def tsit5_placement(n_vars: int, block: int, placement=None) -> dict:
     """The placement macros for a Tsit5 kernel: registers up to
     ``UNROLLED_MAX_VARS``, above it the rolled stage loop with the seven stage

@@ -37,6 +37,11 @@ constexpr int N = MDX_N;
 #ifndef MDX_LDS_K
 #define MDX_LDS_K 0
 #endif
+// Elements per chunk where a loop over the per-lane vectors of the rolled step
+// loads a chunk before computing with it.
+#ifndef MDX_CHUNK
+#define MDX_CHUNK 16
+#endif
 #ifndef MDX_LDS_U
 #define MDX_LDS_U 0
 #endif
@@ -265,18 +270,35 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
 #if !MDX_PARAMS_IN_REGS && MDX_NP > 0
     MDX_PIN(p.v, MDX_NP);
 #endif
+    // a chunk of elements at a time, each chunk's loads ahead of its
+    // arithmetic (the stage sums over every element at once would keep one
+    // accumulator per element live and leave the loads no registers)
     const double* __restrict__ ar = AT[S];
-    double q[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) q[j] = ar[0] * KQ(s, 0, j);
+    for (int j0 = 0; j0 < N; j0 += MDX_CHUNK) {
+      double q[MDX_CHUNK];
+#pragma unroll
+      for (int jj = 0; jj < MDX_CHUNK; ++jj)
+        if (j0 + jj < N) q[jj] = ar[0] * KQ(s, 0, j0 + jj);
 #pragma unroll 1
-    for (int r = 1; r < S; ++r) {
-      const double c = ar[r];
+      for (int r = 1; r < S; ++r) {
+        const double c = ar[r];
+        double kv[MDX_CHUNK];
 #pragma unroll
-      for (int j = 0; j < N; ++j) q[j] = fmaT(c, KQ(s, r, j), q[j]);
+        for (int jj = 0; jj < MDX_CHUNK; ++jj)
+          if (j0 + jj < N) kv[jj] = KQ(s, r, j0 + jj);
+#pragma unroll
+        for (int jj = 0; jj < MDX_CHUNK; ++jj)
+          if (j0 + jj < N) q[jj] = fmaT(c, kv[jj], q[jj]);
+      }
+      double yv[MDX_CHUNK];
+#pragma unroll
+      for (int jj = 0; jj < MDX_CHUNK; ++jj)
+        if (j0 + jj < N) yv[jj] = YJ(s, j0 + jj);
+#pragma unroll
+      for (int jj = 0; jj < MDX_CHUNK; ++jj)
+        if (j0 + jj < N) UJ(j0 + jj) = fmaT(dt_use, q[jj], yv[jj]);
     }
-#pragma unroll
-    for (int j = 0; j < N; ++j) UJ(j) = fmaT(dt_use, q[j], YJ(s, j));
     double out[N];
 #if MDX_LDS_U
     sys::rhs(LdsVec{&lds_u[lane_]}, t + CS[S] * dt_use, p, out);
@@ -288,21 +310,42 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
   }
 #undef MDX_PIN
 
+  // the embedded error, a chunk of elements at a time with each stage's
+  // chunk loaded ahead of its multiply-adds; the norm still sums the elements
+  // in ascending order
   double err = 0.0;
 #pragma unroll
-  for (int j = 0; j < N; ++j) {
-    double e = E1 * KQ(s, 0, j);
-    e = fmaT(E2, KQ(s, 1, j), e);
-    e = fmaT(E3, KQ(s, 2, j), e);
-    e = fmaT(E4, KQ(s, 3, j), e);
-    e = fmaT(E5, KQ(s, 4, j), e);
-    e = fmaT(E6, KQ(s, 5, j), e);
-    e = fmaT(E7, KQ(s, 6, j), e);
-    const double err_est = dt_use * e;
-    const double ya = absT(YJ(s, j)), yb = absT(UJ(j));
-    const double scale = fmaT(a.rtol, ya > yb ? ya : yb, a.atol);
-    const double r = weight(a, i, j) * err_est / scale;
-    err = fmaT(r, r, err);
+  for (int j0 = 0; j0 < N; j0 += MDX_CHUNK) {
+    constexpr double EW[7] = {E1, E2, E3, E4, E5, E6, E7};
+    double e[MDX_CHUNK];
+#pragma unroll
+    for (int jj = 0; jj < MDX_CHUNK; ++jj)
+      if (j0 + jj < N) e[jj] = EW[0] * KQ(s, 0, j0 + jj);
+#pragma unroll
+    for (int r = 1; r < 7; ++r) {
+      double kv[MDX_CHUNK];
+#pragma unroll
+      for (int jj = 0; jj < MDX_CHUNK; ++jj)
+        if (j0 + jj < N) kv[jj] = KQ(s, r, j0 + jj);
+#pragma unroll
+      for (int jj = 0; jj < MDX_CHUNK; ++jj)
+        if (j0 + jj < N) e[jj] = fmaT(EW[r], kv[jj], e[jj]);
+    }
+    double ya[MDX_CHUNK], yb[MDX_CHUNK];
+#pragma unroll
+    for (int jj = 0; jj < MDX_CHUNK; ++jj)
+      if (j0 + jj < N) {
+        ya[jj] = absT(YJ(s, j0 + jj));
+        yb[jj] = absT(UJ(j0 + jj));
+      }
+#pragma unroll
+    for (int jj = 0; jj < MDX_CHUNK; ++jj)
+      if (j0 + jj < N) {
+        const double err_est = dt_use * e[jj];
+        const double scale = fmaT(a.rtol, ya[jj] > yb[jj] ? ya[jj] : yb[jj], a.atol);
+        const double r = weight(a, i, j0 + jj) * err_est / scale;
+        err = fmaT(r, r, err);
+      }
   }
   const double err_norm = __builtin_sqrt(err / (double)MDX_N_ERROR);
   const bool accept = err_norm <= 1.0 && !__builtin_isnan(err_norm);

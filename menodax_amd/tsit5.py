@@ -93,7 +93,7 @@ def solve(
         pcoeff=float(pcoeff),
         icoeff=float(icoeff),
         dcoeff=float(dcoeff),
-        trajectories_per_block=_kernels.trajectories_per_block_or_default(trajectories_per_block),
+        trajectories_per_block=_kernels.tsit5_block(trajectories_per_block, n_vars),
         unit_weights=error_weights is None,
         tableau_in_registers=_tableau(tableau_in_registers, n_vars, n, None, None),
     )
@@ -233,9 +233,9 @@ def _traced(ode_fn, y0, t_span, params, *, rtol=1e-8, atol=1e-10, first_step=Non
         return NotImplemented
     if hasattr(y0, "shape") and len(y0.shape) in (1, 2):
         _check_backend(y0.shape[-1], backend)
+    n_vars = y0.shape[-1] if hasattr(y0, "shape") and len(y0.shape) in (1, 2) else 0
     options = (("pcoeff", float(pcoeff)), ("icoeff", float(icoeff)), ("dcoeff", float(dcoeff)),
-               ("trajectories_per_block",
-                _kernels.trajectories_per_block_or_default(trajectories_per_block)),
+               ("trajectories_per_block", _kernels.tsit5_block(trajectories_per_block, n_vars)),
                ("unit_weights", True))
     return _ops.traced_solve("tsit5", ode_fn, y0, t_span, params, options=options,
                              sparsity=None, tableau=tableau_in_registers, rtol=rtol, atol=atol,

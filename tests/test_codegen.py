@@ -227,6 +227,14 @@ def test_the_tableau_goes_to_registers_for_small_systems_and_ensembles():
     assert "#define MDX_TAB_REGS 0" in rolled
 
 
+def test_a_rolled_tsit5_step_takes_the_largest_block_its_stage_vectors_fit():
+    B = _kernels.tsit5_block
+    assert B(None, 3) == 64 and B(None, 24) == 64 and B(8, 64) == 8
+    assert B(None, 64) == 32 and B(None, 128) == 16 and B(None, 400) == 64
+    assert _kernels.tsit5_placement(64, 32)["MDX_LDS_K"] == 1
+    assert _kernels.tsit5_placement(64, 64)["MDX_LDS_K"] == 0
+
+
 def test_structurally_zero_time_derivative_is_flagged():
     g = cg.generate(cg.trace(S.robertson, 3, 3), colour=[0, 1, 2], n_colours=3,
                     store=[(r, c, r * 3 + c) for r in range(3) for c in range(3)])
