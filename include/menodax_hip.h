@@ -51,6 +51,7 @@ void mdx_free(void* p);
 #define MDX_KERNEL_UNIT_WEIGHTS 2 /* compiled with unit error weights: `weights` unread */
 #define MDX_KERNEL_ADJOINT 4      /* a discrete-adjoint kernel, launched by mdx_adjoint_launch */
 #define MDX_KERNEL_HOOK 8         /* runs a save hook: hook_out (N, hook_size) required */
+#define MDX_KERNEL_WAVE 16        /* one trajectory per workgroup of `block` lanes: grid = N */
 
 /* Register a kernel from a code object: `entry` is its symbol, `block` the
  * workgroup size it was compiled for, `flags` a combination of the

@@ -44,13 +44,20 @@ class _Solver:
     tableau: object        # the caller's tableau_in_registers (None: size rule)
     ckpt_every: int
 
-    def forward(self, n):
+    def forward(self, n, wave_ok=False):
+        """The forward kernels for an ensemble of ``n``; ``wave_ok`` lets the
+        size rule pick the wave placement (a plain launch, no refill)."""
         from .rodas5P import _tableau
 
         opts = replace(self.options, tableau_in_registers=_tableau(
             self.tableau, self.n_vars, n, getattr(self.options, "placement", None),
             getattr(self.options, "max_registers", None)))
         if self.method == "rodas5p":
+            if not wave_ok:
+                return _kernels.rodas5p_kernel(self.ode_fn, self.n_vars, self.n_params, opts)
+            from .rodas5P import _wave
+
+            opts = replace(opts, placement=_wave(opts, self.n_vars, n))
             return _kernels.rodas5p_kernel(self.ode_fn, self.n_vars, self.n_params, opts)
         return _kernels.tsit5_kernel(self.ode_fn, self.n_vars, self.n_params, opts)
 
@@ -98,8 +105,8 @@ def ensemble_solve(y0: torch.Tensor, t_span: torch.Tensor, params: torch.Tensor,
 
     s = _SOLVERS[solver]
     n = y0.shape[0]
-    out = launch(s.forward(n), y0, t_span.contiguous(), params, n, dt0=dt0, rtol=rtol,
-                 atol=atol, max_steps=max_steps, refill=_refill(refill))
+    out = launch(s.forward(n, wave_ok=not refill > 0), y0, t_span.contiguous(), params, n,
+                 dt0=dt0, rtol=rtol, atol=atol, max_steps=max_steps, refill=_refill(refill))
     return out.hist, out.accepted, out.rejected, out.loop
 
 

@@ -25,6 +25,7 @@ KERNEL_PERSISTENT = 1
 KERNEL_UNIT_WEIGHTS = 2
 KERNEL_ADJOINT = 4
 KERNEL_HOOK = 8
+KERNEL_WAVE = 16
 
 # Every kernel is compiled with contraction off (the fused multiply-adds are
 # explicit in the source) and without packed FP32 instructions.

@@ -1,7 +1,8 @@
 """Autotuning the Rodas5P launch: placement, workgroup size, launch shape.
 
-The placements (registers / rolled scratch / rolled LDS), for the unrolled
-step of a small system whether the tableau is held in registers, the
+The placements (registers / rolled scratch / rolled LDS, and for a dense
+matrix above the unrolled size one trajectory per 64-lane workgroup), for the
+unrolled step of a small system whether the tableau is held in registers, the
 trajectories per workgroup and the two launch shapes (one lane per trajectory, persistent
 refill) move data and work, never arithmetic, so every candidate computes the
 same bits. Which is fastest depends on the system size, the register and LDS
@@ -47,12 +48,15 @@ class Choice:
     tableau_in_registers: bool = False
 
 
-def candidates(n_vars: int, lu_precision: str, blocks=(64, 128, 256)):
+def candidates(n_vars: int, lu_precision: str, blocks=(64, 128, 256), dense=False):
     """(placement, block, tableau in registers) worth timing: the unrolled
     register kernel only where it can plausibly fit, with and without the
     tableau in registers where that fits too, the LDS placement only at block
-    sizes where something fits in LDS."""
+    sizes where something fits in LDS, and for a dense matrix above the
+    unrolled size the wave placement (one trajectory per 64-lane workgroup)."""
     out = []
+    if dense and _kernels.DENSE_UNROLL_MAX < n_vars <= _kernels.WAVE_MAX_VARS:
+        out.append((_kernels.WAVE, 64, False))
     for placement in PLACEMENTS:
         if placement == "registers" and n_vars > 32:
             continue
@@ -94,13 +98,16 @@ def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-
     times = _common.as_f64_device(t_span, dev).contiguous()
     kw = dict(dt0=_common.initial_step(first_step), rtol=rtol, atol=atol, max_steps=max_steps)
     results, reference = [], None
-    for placement, block, tab in candidates(n_vars, lu_precision, blocks):
+    for placement, block, tab in candidates(n_vars, lu_precision, blocks,
+                                            dense=sparsity is None):
         built = _kernels.rodas5p_kernel(
             ode_fn, n_vars, n_params,
             replace(base, placement=placement, trajectories_per_block=block,
                     tableau_in_registers=tab))
         launches = [(False, 0), (True, 0)]
-        if n < n_full or shapes == "solve":
+        if not built.refill:
+            launches = [(False, 0)]  # the wave placement has no persistent launch
+        elif n < n_full or shapes == "solve":
             if not _use_refill(built, n_full):
                 launches = [(False, 0)]
             elif n < n_full:

@@ -96,6 +96,7 @@ struct Kernel {
   bool unit_weights = false;
   bool adjoint = false;
   bool hook = false;
+  bool wave = false;  // one trajectory per workgroup
   std::map<int, Loaded> per_device;
 };
 
@@ -205,8 +206,11 @@ int32_t mdx_kernel_create(const void* code, size_t code_size, const char* entry,
   k->unit_weights = (flags & MDX_KERNEL_UNIT_WEIGHTS) != 0;
   k->adjoint = (flags & MDX_KERNEL_ADJOINT) != 0;
   k->hook = (flags & MDX_KERNEL_HOOK) != 0;
+  k->wave = (flags & MDX_KERNEL_WAVE) != 0;
   if (k->adjoint && k->persistent)
     return fail(1, "mdx_kernel_create: an adjoint kernel is not a persistent kernel");
+  if (k->wave && (k->persistent || k->adjoint))
+    return fail(1, "mdx_kernel_create: a wave-per-trajectory kernel is a plain solve kernel");
   std::lock_guard<std::mutex> lock(g_mutex);
   g_kernels.push_back(std::move(k));
   *handle = (int64_t)g_kernels.size();
@@ -339,9 +343,10 @@ int32_t launch_impl(int64_t handle, void* stream, int32_t grid, int64_t n_traj, 
     return fail(1, "mdx_solve_launch: this kernel runs a save hook; hook_out and hook_size are required");
   if (max_steps < 0) return fail(1, "mdx_solve_launch: max_steps must be non-negative");
   if (k->adjoint) return fail(1, "mdx_solve_launch: an adjoint kernel is launched by mdx_adjoint_launch");
-  if (grid > 0 && !k->persistent && (int64_t)grid * k->block < n_traj)
+  const int64_t per_group = k->wave ? 1 : k->block;  // trajectories per workgroup
+  if (grid > 0 && !k->persistent && (int64_t)grid * per_group < n_traj)
     return fail(1, "mdx_solve_launch: a one-lane-per-trajectory launch needs grid >= "
-                   "ceil(n_traj / block)");
+                   "ceil(n_traj / block) (n_traj for a wave-per-trajectory kernel)");
   if (order && !k->persistent)
     return fail(1, "mdx_solve_launch_ordered: a claim order needs a persistent (refill) kernel");
   if (cost && (k->persistent || k->hook))
@@ -407,7 +412,7 @@ int32_t launch_impl(int64_t handle, void* stream, int32_t grid, int64_t n_traj, 
       int64_t needed = (n_traj + k->block - 1) / k->block;
       if (blocks > needed) blocks = needed;
     } else {
-      blocks = (n_traj + k->block - 1) / k->block;
+      blocks = (n_traj + per_group - 1) / per_group;
     }
   }
   if (blocks > 0x7fffffff) return fail(1, "mdx_solve_launch: grid too large");

@@ -183,7 +183,31 @@ struct Shifted {
 // every LDS address is that one register plus an immediate offset; otherwise
 // the compiler hoists hundreds of per-element addresses out of the step loop
 // and spills them.
-#if MDX_LDS_K || MDX_LDS_Y || MDX_LDS_U || MDX_LDS_LU
+// One trajectory per workgroup (MDX_COOP, the "wave" placement): every lane
+// runs the step on the same values, all per-trajectory arrays live once in
+// LDS, and the lanes share out the dense factorisation and the forward
+// substitution (coop_factor, coop_solve). Otherwise each lane is a trajectory
+// and the LDS arrays interleave the lanes' copies.
+#ifndef MDX_COOP
+#define MDX_COOP 0
+#endif
+#if MDX_COOP && !(MDX_ROLLED && MDX_LDS_K && MDX_LDS_U && MDX_LDS_Y && MDX_LDS_LU)
+#error "a wave-per-trajectory kernel keeps every vector and the matrix in LDS"
+#endif
+#if MDX_COOP && (MDX_NSENS > 0 || MDX_HOOK || MDX_ADJOINT || MDX_BLOCK != 64)
+#error "a wave-per-trajectory kernel is the plain forward solve on one 64-lane wave"
+#endif
+#if MDX_COOP
+#define MDX_LSTRIDE 1
+// row stride of the shared matrix: one padding element keeps a column's
+// elements in different banks
+#define MDX_LD (N + 1)
+#else
+#define MDX_LSTRIDE MDX_BLOCK
+#endif
+#if MDX_COOP
+#define MDX_LANE() constexpr unsigned lane_ = 0
+#elif MDX_LDS_K || MDX_LDS_Y || MDX_LDS_U || MDX_LDS_LU
 __device__ __forceinline__ unsigned opaque_lane() {
   unsigned v = threadIdx.x;
   asm volatile("; lane" : "+v"(v));
@@ -194,24 +218,38 @@ __device__ __forceinline__ unsigned opaque_lane() {
 #define MDX_LANE() (void)0
 #endif
 #if MDX_LDS_K
-__shared__ LUT lds_k[8 * NA * MDX_BLOCK];
-#define KQ(q, j) lds_k[((q) * NA + (j)) * MDX_BLOCK + lane_]
+__shared__ LUT lds_k[8 * NA * MDX_LSTRIDE];
+#define KQ(q, j) lds_k[((q) * NA + (j)) * MDX_LSTRIDE + lane_]
 #else
 #define KQ(q, j) k[q][j]
 #endif
 #if MDX_LDS_Y
-__shared__ double lds_y[NA * MDX_BLOCK];
-#define YJ(st, j) lds_y[(j) * MDX_BLOCK + lane_]
+__shared__ double lds_y[NA * MDX_LSTRIDE];
+#define YJ(st, j) lds_y[(j) * MDX_LSTRIDE + lane_]
 #else
 #define YJ(st, j) (st).y[j]
 #endif
 #if MDX_LDS_U
-__shared__ double lds_u[NA * MDX_BLOCK];
-#define UJ(j) lds_u[(j) * MDX_BLOCK + lane_]
+__shared__ double lds_u[NA * MDX_LSTRIDE];
+#define UJ(j) lds_u[(j) * MDX_LSTRIDE + lane_]
 #else
 #define UJ(j) u[j]
 #endif
-#if MDX_LDS_LU
+#if MDX_LDS_LU && MDX_COOP
+// The trajectory's dense iteration matrix, row-major with padded rows;
+// element e = row * N + col as the generated code numbers it.
+__shared__ LUT lds_lu[N * MDX_LD];
+__shared__ LUT lds_r[N];
+__shared__ int lds_piv[N];
+__shared__ double lds_rd[N];
+struct LdsMat {
+  typedef LUT value_type;
+  LUT* base;
+  __device__ __forceinline__ LUT& operator[](int e) const {
+    return base[(e / N) * MDX_LD + e % N];
+  }
+};
+#elif MDX_LDS_LU
 // The dense iteration matrix in LDS, element e of this lane at [e][lane]: the
 // factorisation's loads then take LDS latency instead of scratch latency.
 __shared__ LUT lds_lu[MDX_NLU * MDX_BLOCK];
@@ -224,7 +262,7 @@ struct LdsMat {
 // A vector in LDS as the generated right-hand side reads it.
 struct LdsVec {
   const double* base;
-  __device__ __forceinline__ double operator[](int j) const { return base[j * MDX_BLOCK]; }
+  __device__ __forceinline__ double operator[](int j) const { return base[j * MDX_LSTRIDE]; }
 };
 #if MDX_DT_ZERO
 #define DTJ(j) 0.0
@@ -548,11 +586,143 @@ __device__ __forceinline__ int diag_slot(int d) {
 #endif
 }
 
+#if MDX_COOP
+// The dense factorisation shared out over the trajectory's 64 lanes, with the
+// matrix in LDS (rows of MDX_LD elements). Per pivot i:
+// - the pivot is the first maximum of |a_ki| over k >= i, as the sequential
+//   scan finds it (a NaN below the diagonal never wins, a NaN diagonal keeps
+//   row i): each lane offers its row, and a butterfly over the wave keeps the
+//   larger value, or on a tie the smaller row;
+// - the two rows are swapped a column per lane;
+// - for each row k below, every lane forms the same multiplier
+//   f = a_ki * (1/u_ii) in double and updates its own column j > i with
+//   a_kj = (T)fma(-f, u_ij, a_kj), the sequential update of that element;
+// - column i then takes the multipliers, a row per lane.
+// Every element sees the operations of the sequential factorisation in the
+// same order, so the bits are the same.
+__device__ __forceinline__ void coop_factor(LUT* __restrict__ a) {
+  typedef LUT T;
+  const int c = (int)threadIdx.x;
+#pragma unroll 1
+  for (int i = 0; i < N; ++i) {
+    int piv = i;
+    const T dii = absT(a[i * MDX_LD + i]);
+    if (dii == dii) {
+      T v = (T)-1;
+      if (c >= i && c < N) {
+        v = absT(a[c * MDX_LD + i]);
+        if (v != v) v = (T)-1;
+      }
+      int idx = c;
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) {
+        const T ov = __shfl_xor(v, m);
+        const int oi = __shfl_xor(idx, m);
+        if (ov > v || (ov == v && oi < idx)) {
+          v = ov;
+          idx = oi;
+        }
+      }
+      piv = __builtin_amdgcn_readfirstlane(idx);
+    }
+    if (c == 0) lds_piv[i] = piv;
+    if (piv != i && c < N) {
+      const T x = a[i * MDX_LD + c];
+      a[i * MDX_LD + c] = a[piv * MDX_LD + c];
+      a[piv * MDX_LD + c] = x;
+    }
+    __syncthreads();
+    const T pv = a[i * MDX_LD + i];
+    const double inv = 1.0 / (double)pv;
+    if (c == 0) lds_rd[i] = inv;
+    if (pv != (T)0) {
+      const bool mine = c > i && c < N;
+      const double ui = mine ? (double)a[i * MDX_LD + c] : 0.0;
+#pragma unroll 4
+      for (int k = i + 1; k < N; ++k) {
+        const double f = (double)a[k * MDX_LD + i] * inv;
+        if (mine) a[k * MDX_LD + c] = (T)fmaT(-f, ui, (double)a[k * MDX_LD + c]);
+      }
+      __syncthreads();
+      if (mine) a[c * MDX_LD + i] = (T)((double)a[c * MDX_LD + i] * inv);
+    }
+    __syncthreads();
+  }
+}
+
+// Solve with coop_factor's factorisation, the right-hand side in registers
+// (r, the same on every lane): the row swaps in order, the forward
+// substitution a row per lane (row c takes column j's term once row j is
+// final, so each row sums in ascending column order, as sequentially), then
+// the back substitution as one chain, every lane computing the same values.
+__device__ __forceinline__ void coop_solve(const LUT* __restrict__ a, LUT* __restrict__ r) {
+  typedef LUT T;
+  const int c = (int)threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < N; ++j) lds_r[j] = r[j];
+  __syncthreads();
+#pragma unroll 1
+  for (int i = 0; i < N; ++i) {
+    const int piv = lds_piv[i];
+    if (piv != i) {
+      const T x = lds_r[i];
+      const T y = lds_r[piv];
+      lds_r[i] = y;
+      lds_r[piv] = x;
+    }
+  }
+  __syncthreads();
+  T acc = c < N ? lds_r[c] : (T)0;
+#pragma unroll 1
+  for (int j0 = 0; j0 < N - 1; j0 += MDX_CHUNK) {
+    T l[MDX_CHUNK];
+#pragma unroll
+    for (int q = 0; q < MDX_CHUNK; ++q)
+      l[q] = c < N ? a[c * MDX_LD + clampN(j0 + q)] : (T)0;
+#pragma unroll
+    for (int q = 0; q < MDX_CHUNK; ++q) {
+      const int j = j0 + q;
+      if (j < N - 1) {
+        const T rj = __shfl(acc, j);
+        if (c > j && c < N) acc = fmaT(-l[q], rj, acc);
+      }
+    }
+  }
+  if (c < N) lds_r[c] = acc;
+  __syncthreads();
+#pragma unroll 1
+  for (int i = N - 1; i >= 0; --i) {
+    T x = lds_r[i];
+#pragma unroll 1
+    for (int j0 = i + 1; j0 < N; j0 += MDX_CHUNK) {
+      T uv[MDX_CHUNK], rv[MDX_CHUNK];
+#pragma unroll
+      for (int q = 0; q < MDX_CHUNK; ++q) {
+        uv[q] = a[i * MDX_LD + clampN(j0 + q)];
+        rv[q] = lds_r[clampN(j0 + q)];
+      }
+#pragma unroll
+      for (int q = 0; q < MDX_CHUNK; ++q)
+        if (j0 + q < N) x = fmaT(-uv[q], rv[q], x);
+    }
+    lds_r[i] = x / a[i * MDX_LD + i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < N; ++j) r[j] = lds_r[j];
+  __syncthreads();
+}
+#endif
+
 template <class L>
 __device__ __forceinline__ void factor(L lu, Pivots& ipiv) {
 #if MDX_SPARSE
   (void)ipiv;
   sys::sparse_factor(lu);
+#elif MDX_COOP
+  (void)ipiv;
+  __syncthreads();
+  coop_factor(lu.base);
 #else
   dense_factor(lu, ipiv);
 #endif
@@ -563,6 +733,9 @@ __device__ __forceinline__ void solve(const L& lu, const Pivots& ipiv, LUT* __re
 #if MDX_SPARSE
   (void)ipiv;
   sys::sparse_solve(lu, r);
+#elif MDX_COOP
+  (void)ipiv;
+  coop_solve(lu.base, r);
 #else
   dense_solve(lu, ipiv, r);
 #endif
@@ -1152,7 +1325,12 @@ __device__ __forceinline__ void finish(State& s, const PA& p, const SolveArgs& a
 
 extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p(mdx::SolveArgs a) {
   using namespace mdx::r5;
+#if MDX_COOP
+  // one trajectory per workgroup: every lane steps it, sharing the LU
+  const mdx::i64 i = (mdx::i64)blockIdx.x;
+#else
   const mdx::i64 i = (mdx::i64)blockIdx.x * MDX_BLOCK + threadIdx.x;
+#endif
   if (i >= a.n_traj) return;
   PA p;
   p.load(a, i);
@@ -1163,6 +1341,7 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p(
   finish(s, p, a, i);
 }
 
+#if !MDX_COOP
 // Persistent refill: a resident grid whose lanes claim trajectories from a
 // device counter and take the next one as soon as theirs ends, so a wave keeps
 // every lane busy while step counts across the ensemble differ by orders of
@@ -1204,4 +1383,5 @@ extern "C" __global__ __launch_bounds__(MDX_BLOCK) MDX_REG_CAP void mdx_rodas5p_
     }
   }
 }
+#endif  // !MDX_COOP
 #endif  // !MDX_ADJOINT

@@ -235,6 +235,25 @@ def test_a_rolled_tsit5_step_takes_the_largest_block_its_stage_vectors_fit():
     assert _kernels.tsit5_placement(64, 64)["MDX_LDS_K"] == 0
 
 
+def test_the_wave_placement_is_for_small_ensembles_of_dense_systems():
+    W = _kernels.wave_placement
+    assert W(64, 1000, 256) == "wave" and W(24, 1024, 256) == "wave"
+    assert W(64, 1025, 256) is None and W(16, 10, 256) is None and W(66, 10, 256) is None
+    assert W(64, 10, 256, "lds") == "lds" and W(64, 10, 256, None, sparsity=((0,),)) is None
+    fn, _ = S.vdp_lattice(12)
+    src, d = _kernels.rodas5p_source(fn, 24, 1, _kernels.KernelOptions(placement="wave"))
+    assert "#define MDX_COOP 1" in src and d["block"] == 64
+    assert _runtime.rtc_compile(src, "ring_wave.hip")[:4] == b"\x7fELF"
+    # the adjoint of a wave solve runs the rolled step
+    adj, _ = _kernels.rodas5p_source(fn, 24, 1, _kernels.KernelOptions(placement="wave"),
+                                     adjoint=True)
+    assert "#define MDX_COOP 0" in adj
+    with pytest.raises(ValueError, match="wave"):
+        _kernels.rodas5p_placement(24, "fp32", 64, "wave", dense_matrix=0)
+    with pytest.raises(ValueError, match="wave"):
+        _kernels.rodas5p_placement(80, "fp32", 64, "wave", dense_matrix=6400)
+
+
 def test_structurally_zero_time_derivative_is_flagged():
     g = cg.generate(cg.trace(S.robertson, 3, 3), colour=[0, 1, 2], n_colours=3,
                     store=[(r, c, r * 3 + c) for r in range(3) for c in range(3)])

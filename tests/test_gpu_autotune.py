@@ -50,3 +50,15 @@ def test_a_small_system_is_timed_with_and_without_the_tableau_in_registers():
     opts = autotune.tuned_options(S.robertson, y0, t, p, blocks=(64,), **kw)
     np.testing.assert_array_equal(rodas5P.solve(S.robertson, y0, t, p, **opts, **kw),
                                   rodas5P.solve(S.robertson, y0, t, p, **kw))
+
+
+def test_a_dense_system_above_the_unrolled_size_is_timed_in_the_wave_placement():
+    fn, _ = S.vdp_lattice(12)
+    y0, p = S.vdp_scenario(12, 256, 3, divergence=0.25)
+    kw = dict(first_step=1e-4, rtol=1e-6, atol=1e-8, lu_precision="fp32")
+    best, table = autotune.tune(fn, y0, S.VDP_TIMES, p, blocks=(16, 64), **kw)
+    waves = [c for c in table if c.placement == "wave"]
+    assert len(waves) == 1 and not waves[0].refill
+    opts = autotune.tuned_options(fn, y0, S.VDP_TIMES, p, blocks=(16, 64), **kw)
+    np.testing.assert_array_equal(rodas5P.solve(fn, y0, S.VDP_TIMES, p, **opts, **kw),
+                                  rodas5P.solve(fn, y0, S.VDP_TIMES, p, **kw))

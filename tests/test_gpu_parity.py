@@ -314,6 +314,23 @@ def test_subnormal_stage_quotients_match_the_divisions(tab):
     _same(hist, st, ref, rst)
 
 
+@pytest.mark.parametrize("lu", ["fp32", "fp64"])
+@pytest.mark.parametrize("n_osc", [12, 20])
+def test_the_wave_placement_is_bit_exact(lu, n_osc):
+    """One trajectory per 64-lane workgroup, the lanes sharing out the dense
+    factorisation and forward substitution: every trajectory matches the
+    oracle bit for bit; for these ensemble sizes it is also the size rule's
+    default placement."""
+    fn, _ = S.vdp_lattice(n_osc)
+    y0, p = S.vdp_scenario(n_osc, 300, 11, divergence=0.25)
+    kw = dict(first_step=1e-4, rtol=1e-6, atol=1e-8, lu_precision=lu, return_stats=True)
+    ref, rst = O.rodas5p(O.vdp_lattice(n_osc), y0, S.VDP_TIMES, p, first_step=1e-4, rtol=1e-6,
+                         atol=1e-8, lu_precision=lu)
+    for placement in ("wave", None):
+        hist, st = rodas5P.solve(fn, y0, S.VDP_TIMES, p, placement=placement, **kw)
+        _same(hist, st, ref, rst)
+
+
 def test_a_claim_order_changes_no_bit():
     """The persistent launch handing trajectories out in a given order -- the
     costliest first, shuffled, or ranked by a short pilot -- computes every
