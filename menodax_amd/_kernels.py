@@ -10,6 +10,8 @@ trajectory and the persistent refill kernel -- come from the same code object.
 
 from __future__ import annotations
 
+import math
+
 import functools
 from dataclasses import dataclass, field, replace
 
@@ -145,12 +147,15 @@ def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None
     if placement is None:
         placement = "registers" if n_vars <= UNROLLED_MAX_VARS else "lds"
     if placement == WAVE:
-        if not dense_matrix or n_vars > WAVE_MAX_VARS:
+        n = math.isqrt(dense_matrix)
+        if not DENSE_UNROLL_MAX < n <= WAVE_MAX_VARS:
             raise ValueError("the wave placement is for a dense matrix of "
                              f"{DENSE_UNROLL_MAX + 1}..{WAVE_MAX_VARS} variables (the forward solve)")
         lu_bytes = 4 if lu_precision == "fp32" else 8
-        lds = (8 * n_vars * lu_bytes + 16 * n_vars + n_vars * (n_vars + 1) * lu_bytes
-               + n_vars * (lu_bytes + 12))
+        # stage vectors, stage argument and state (n_vars long: the joint
+        # vector of a sensitivity solve), the padded matrix, its pivots
+        lds = (8 * n_vars * lu_bytes + 16 * n_vars + n * (n + 1) * lu_bytes
+               + n * (lu_bytes + 12))
         return {"MDX_ROLLED": 1, "MDX_LDS_LU": 1, "MDX_LDS_K": 1, "MDX_LDS_U": 1,
                 "MDX_LDS_Y": 1, "MDX_COOP": 1, "lds_bytes": lds, "placement": WAVE}
     if placement not in PLACEMENTS:
@@ -332,8 +337,8 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
     if spec is not None and (spec.n_vars, spec.n_params) != (n_vars, n_params):
         raise ValueError("the sensitivity spec was made for another system size")
     if options.placement == WAVE and not adjoint:
-        if spec is not None or options.save_hook is not None:
-            raise ValueError("the wave placement runs the plain forward solve")
+        if options.save_hook is not None:
+            raise ValueError("the wave placement runs no save hook")
         block = 64  # lanes of the one workgroup a trajectory takes
     n_aug = n_vars if spec is None else spec.n_aug
     if adjoint and spec is not None:

@@ -194,8 +194,8 @@ struct Shifted {
 #if MDX_COOP && !(MDX_ROLLED && MDX_LDS_K && MDX_LDS_U && MDX_LDS_Y && MDX_LDS_LU)
 #error "a wave-per-trajectory kernel keeps every vector and the matrix in LDS"
 #endif
-#if MDX_COOP && (MDX_NSENS > 0 || MDX_HOOK || MDX_ADJOINT || MDX_BLOCK != 64)
-#error "a wave-per-trajectory kernel is the plain forward solve on one 64-lane wave"
+#if MDX_COOP && (MDX_HOOK || MDX_ADJOINT || MDX_BLOCK != 64)
+#error "a wave-per-trajectory kernel is a forward solve (plain or joint) on one 64-lane wave"
 #endif
 #if MDX_COOP
 #define MDX_LSTRIDE 1

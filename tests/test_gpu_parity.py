@@ -331,6 +331,20 @@ def test_the_wave_placement_is_bit_exact(lu, n_osc):
         _same(hist, st, ref, rst)
 
 
+@pytest.mark.parametrize("lu", ["fp32", "fp64"])
+def test_the_wave_placement_of_a_joint_sensitivity_solve_changes_no_bit(lu):
+    """The joint [y, S] solve (sensitivities in the parameter) on one 64-lane
+    workgroup per trajectory, the size rule's choice here, against the rolled
+    per-lane kernel: states and sensitivities bit-identical."""
+    fn, _ = S.vdp_lattice(12)
+    y0, p = S.vdp_scenario(12, 200, 5, divergence=0.25)
+    kw = dict(first_step=1e-4, rtol=1e-6, atol=1e-8, lu_precision=lu, wrt_y0=False)
+    got = rodas5P.solve_sensitivities(fn, y0, S.VDP_TIMES, p, **kw)
+    want = rodas5P.solve_sensitivities(fn, y0, S.VDP_TIMES, p, placement="rolled", **kw)
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_a_claim_order_changes_no_bit():
     """The persistent launch handing trajectories out in a given order -- the
     costliest first, shuffled, or ranked by a short pilot -- computes every
