@@ -597,6 +597,34 @@ def reference_benchmarks(torch, log_fn, reps=5):
     return out
 
 
+def wave_ring(torch, log_fn, reps=3):
+    """The reference's 64-variable stiff Van der Pol ring, 1,000 identical
+    trajectories, dense LU: the per-lane kernel (placement "lds") against the
+    wave placement, FP32 and FP64 LU, one warm-up and ``reps`` timed solves."""
+    from menodax_amd import rodas5P, systems as S
+
+    dev = torch.device("cuda", 0)
+    fn, _ = S.vdp_lattice(32)
+    y0n, pn = S.vdp_scenario(32, 1000, divergence=0.0)
+    y0, p = torch.tensor(y0n, device=dev), torch.tensor(pn, device=dev)
+    t = torch.tensor(S.VDP_TIMES, device=dev)
+    out = {}
+    for lu in ("fp32", "fp64"):
+        for placement in ("lds", "wave"):
+            kw = dict(first_step=1e-4, rtol=1e-6, atol=1e-8, lu_precision=lu,
+                      placement=placement)
+            rodas5P.solve(fn, y0, t, p, **kw)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                rodas5P.solve(fn, y0, t, p, **kw)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            out[f"{lu}_{placement}_ms"] = ms
+            log_fn(f"wave ring: {lu} {placement} {ms:.2f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -614,6 +642,9 @@ def main():
                     help="leave out the adjoint-gradient block")
     ap.add_argument("--vdp-chain", action="store_true",
                     help="also run the 64-variable Van der Pol chain ensemble (seconds to minutes)")
+    ap.add_argument("--only-wave-ring", action="store_true",
+                    help="run and print only the 64-variable dense Van der Pol ring, per-lane "
+                         "kernel against the wave placement (for profiling)")
     ap.add_argument("--only-uneven-stiff", action="store_true",
                     help="run and print only the uneven stiff ensemble comparison (for profiling)")
     ap.add_argument("--uneven-variants", default=",".join(UNEVEN_VARIANTS),
@@ -634,7 +665,8 @@ def main():
         sys.exit(2)
 
     scipy_pool = None
-    if not (args.no_cpu_baseline or args.only_uneven_stiff or args.calibrate_copy
+    if not (args.no_cpu_baseline or args.only_uneven_stiff or args.only_wave_ring
+            or args.calibrate_copy
             or args.lu != "both"):
         # worker processes for the SciPy CPU baseline, spawned while this
         # process has not yet initialised the GPU
@@ -646,6 +678,10 @@ def main():
     import torch
 
     from menodax_amd import _kernels, _runtime, roofline, rodas5P, systems as S
+
+    if args.only_wave_ring:
+        print(json.dumps({"wave_ring": wave_ring(torch, log)}), flush=True)
+        return
 
     if args.only_uneven_stiff:
         from oracle import oracle as O
