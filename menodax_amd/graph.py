@@ -15,8 +15,10 @@ Replays give the same bits as the eager solve (tests/test_gpu_graph.py).
 
 from __future__ import annotations
 
+from dataclasses import replace
+
 from . import _common, _kernels
-from .rodas5P import _tableau, launch
+from .rodas5P import _tableau, _wave, launch
 from ._kernels import KernelOptions, trajectories_per_block_or_default
 from ._sparsity import normalize_sparsity
 
@@ -48,6 +50,8 @@ class CapturedSolve:
             placement=placement,
             tableau_in_registers=_tableau(tableau_in_registers, n_vars, n, placement,
                                           max_registers))
+        if work_order is None and not refill:
+            options = replace(options, placement=_wave(options, n_vars, n))
         self.n, self.n_vars, self.n_params = n, n_vars, p_t.shape[1]
         self.built = _kernels.rodas5p_kernel(ode_fn, n_vars, self.n_params, options)
         # static, structure-of-arrays inputs the graph reads

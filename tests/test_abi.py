@@ -120,6 +120,10 @@ def test_an_empty_ensemble_launches_nothing(handles):
 def test_kernel_create_validates():
     with pytest.raises(_runtime.NativeError, match="block must be"):
         _runtime.kernel_create(b"\x7fELF....", "x", 0, 0)
+    # a wave-per-trajectory kernel is a plain solve kernel
+    for other in (_runtime.KERNEL_PERSISTENT, _runtime.KERNEL_ADJOINT):
+        with pytest.raises(_runtime.NativeError, match="wave-per-trajectory"):
+            _runtime.kernel_create(b"\x7fELF....", "x", 64, _runtime.KERNEL_WAVE | other)
 
 
 def test_compile_returns_a_gfx950_code_object():

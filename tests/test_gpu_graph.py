@@ -61,3 +61,23 @@ def test_a_claim_order_set_between_replays():
         np.testing.assert_array_equal(cs.run(), eager)
     with pytest.raises(ValueError, match="permutation"):
         cs.set_work_order(np.zeros(4096, dtype=np.int64))
+
+
+def test_a_wave_placement_solve_replays_bit_identically():
+    """A small ensemble of a 24-variable dense system takes the wave
+    placement (one trajectory per 64-lane workgroup, with barriers inside the
+    kernel); its captured replays match the eager solve with new inputs too."""
+    import torch
+
+    fn, _ = S.vdp_lattice(12)
+    y0, p = S.vdp_scenario(12, 200, 8, divergence=0.25)
+    y0b, pb = S.vdp_scenario(12, 200, 9, divergence=0.25)
+    kw = dict(first_step=1e-4, rtol=1e-6, atol=1e-8)
+    cs = CapturedSolve(fn, y0, S.VDP_TIMES, p, **kw)
+    assert cs.built.extra["placement"]["placement"] == "wave"
+    eager = rodas5P.solve(fn, y0, S.VDP_TIMES, p, placement="rolled", **kw)
+    for _ in range(2):
+        np.testing.assert_array_equal(cs.run(), eager)
+        torch.randn(1 << 16, device="cuda", dtype=torch.float64).cumsum(0).sum().item()
+    np.testing.assert_array_equal(cs.run(y0=y0b, params=pb),
+                                  rodas5P.solve(fn, y0b, S.VDP_TIMES, pb, placement="rolled", **kw))
