@@ -331,6 +331,45 @@ def test_the_wave_placement_is_bit_exact(lu, n_osc):
         _same(hist, st, ref, rst)
 
 
+def _pivoting_blocks(n=20, size=200, seed=4):
+    """dy/dt = A(p) y + 0.01 y*y with 2x2 blocks [[p_j, 1], [-b'_k, p_j]] (b' =
+    LogUniform[1e2, 1e6]) chained by 0.5 couplings: for a fast block the
+    iteration matrix's subdiagonal b' exceeds its diagonal 1/(h gamma) + |p_j|,
+    so the dense factorisation swaps rows, for some blocks and not others."""
+    rows, cols, vals, pidx = [], [], [], []
+    for k in range(n // 2):
+        rows += [2 * k, 2 * k + 1, 2 * k]
+        cols += [2 * k + 1, 2 * k, (2 * k + 2) % n]
+        vals += [1.0, -1.0, 0.5]
+        pidx += [-1, n + k, -1]
+    terms = [[f"p[{j}] * y[{j}]"] for j in range(n)]
+    for r, c, v, k in zip(rows, cols, vals, pidx):
+        terms[r].append(("-" if v < 0 else "") + f"p[{k}] * y[{c}]" if k >= 0
+                        else f"{float(v)!r} * y[{c}]")
+    fn = S.make_tuple_callback(
+        "pivoting", [" + ".join(t) + f" + 0.01 * (y[{j}] * y[{j}])" for j, t in enumerate(terms)])
+    rng = np.random.default_rng(seed)
+    p = np.concatenate([-rng.uniform(5.0, 20.0, (size, n)),
+                        10.0 ** rng.uniform(2.0, 6.0, (size, n // 2))], axis=1)
+    y0 = rng.uniform(-1.0, 1.0, (size, n))
+    system = O.linquad(n, rows, cols, vals, 0.01, off_pidx=pidx, n_p=n + n // 2)
+    return fn, system, y0, p
+
+
+@pytest.mark.parametrize("lu", ["fp32", "fp64"])
+def test_the_wave_placement_pivots_as_the_sequential_scan(lu):
+    """A dense system whose factorisation swaps rows (a different set per
+    trajectory and step): the wave placement's butterfly pivot search, row
+    swaps and shared updates match the oracle's sequential LU bit for bit."""
+    fn, system, y0, p = _pivoting_blocks()
+    t = np.linspace(0.0, 0.2, 5)
+    ref, rst = O.rodas5p(system, y0, t, p, rtol=1e-6, atol=1e-8, lu_precision=lu)
+    for placement in ("wave", "rolled"):
+        hist, st = rodas5P.solve(fn, y0, t, p, rtol=1e-6, atol=1e-8, lu_precision=lu,
+                                 placement=placement, return_stats=True)
+        _same(hist, st, ref, rst)
+
+
 @pytest.mark.parametrize("lu", ["fp32", "fp64"])
 def test_the_wave_placement_of_a_joint_sensitivity_solve_changes_no_bit(lu):
     """The joint [y, S] solve (sensitivities in the parameter) on one 64-lane
