@@ -1110,7 +1110,7 @@ def generate_rhs(traced: Traced):
     return src, fc
 
 
-def generate_neg_jac(traced: Traced, *, colour, n_colours, store):
+def generate_neg_jac(traced: Traced, *, colour, n_colours, store, skip_zero=False):
     """``neg_jac(y, t, p, lu, dT)``: one forward sweep per colour writing
     ``lu[slot] = -J`` for every ``(row, colour, slot)`` in ``store`` (the order
     the reference writes them, modax/rodas5P.py:278-296), then the ``t``-seeded
@@ -1124,7 +1124,10 @@ def generate_neg_jac(traced: Traced, *, colour, n_colours, store):
     dt_tan = tangents(g, traced.outputs, {g.t().id: 1.0})
     roots = [n for sw in sweeps for n in sw if n is not None] + [n for n in dt_tan if n is not None]
     lines, fc = Emitter(g).body(roots)
-    stores = [f"  lu[{slot}] = static_cast<LUT>(-{value_of(sweeps[col][row])});" for row, col, slot in store]
+    # skip_zero: a slot whose derivative is structurally zero is left to a
+    # clear of the whole buffer to -0.0 (the value its store would write)
+    stores = [f"  lu[{slot}] = static_cast<LUT>(-{value_of(sweeps[col][row])});"
+              for row, col, slot in store if not (skip_zero and sweeps[col][row] is None)]
     dT_lines = [f"  dT[{j}] = {value_of(n)};" for j, n in enumerate(dt_tan)]
     src = "\n".join(
         [
@@ -1297,10 +1300,11 @@ def sensitivity_from_graph(traced: Traced, y, t, p, s, pd, v=None):
     return ev(sn.rhs), ev(sn.dt), ev(sn.coupling)
 
 
-def generate(traced: Traced, *, colour, n_colours, store):
+def generate(traced: Traced, *, colour, n_colours, store, skip_zero=False):
     """``rhs`` and ``neg_jac`` for ``traced`` (see the two generators)."""
     rhs_src, rhs_fc = generate_rhs(traced)
-    jac_src, jac_fc = generate_neg_jac(traced, colour=colour, n_colours=n_colours, store=store)
+    jac_src, jac_fc = generate_neg_jac(traced, colour=colour, n_colours=n_colours, store=store,
+                                       skip_zero=skip_zero)
     g = traced.graph
     dt_zero = all(n is None for n in tangents(g, traced.outputs, {g.t().id: 1.0}))
     return GeneratedSystem(

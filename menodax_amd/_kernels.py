@@ -317,9 +317,12 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
         sparse_src = solver.device_source()
         m, u, s = layout.flop_counts()
         lu_f, solve_f = m + 2 * u + n_vars, 2 * s + n_vars
+    # the wave placement clears its shared matrix a column per lane, so the
+    # Jacobian writes only its structurally nonzero slots
+    wave = (options.placement == WAVE and not adjoint and options.sparsity is None)
     gen = _codegen.generate(
         traced, colour=structure.colour, n_colours=structure.n_colours,
-        store=structure.stores(),
+        store=structure.stores(), skip_zero=wave,
     )
     if options.array_rhs is not None and options.sens is None:
         # the joint system's rows come from ode_fn's sweeps, as in the reference
@@ -327,7 +330,7 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
         rhs_src, rhs_fc = _codegen.generate_rhs(arr)
         jac_src, _ = _codegen.generate_neg_jac(
             traced, colour=structure.colour, n_colours=structure.n_colours,
-            store=structure.stores())
+            store=structure.stores(), skip_zero=wave)
         gen = replace(gen, source=rhs_src + "\n\n" + jac_src + "\n", rhs_flops=rhs_fc)
     if options.tf_index >= n_params:
         raise ValueError(f"tf_index {options.tf_index} is not a column of the {n_params} parameters")

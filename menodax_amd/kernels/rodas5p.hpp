@@ -241,6 +241,7 @@ __shared__ double lds_u[NA * MDX_LSTRIDE];
 __shared__ LUT lds_lu[N * MDX_LD];
 __shared__ LUT lds_r[N];
 __shared__ int lds_piv[N];
+__shared__ int lds_perm[N];
 __shared__ double lds_rd[N];
 struct LdsMat {
   typedef LUT value_type;
@@ -600,6 +601,28 @@ __device__ __forceinline__ int diag_slot(int d) {
 // - column i then takes the multipliers, a row per lane.
 // Every element sees the operations of the sequential factorisation in the
 // same order, so the bits are the same.
+// Lane l's value of v, read by every lane (l uniform): a VALU read across the
+// wave, with no LDS round trip on the path.
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ void coop_clear(LUT* __restrict__ a) {
+  const int c = (int)threadIdx.x;
+  __syncthreads();
+  if (c < N) {
+#pragma unroll 1
+    for (int r = 0; r < N; ++r) a[r * MDX_LD + c] = (LUT)-0.0;
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ void coop_factor(LUT* __restrict__ a) {
   typedef LUT T;
   const int c = (int)threadIdx.x;
@@ -637,21 +660,40 @@ __device__ __forceinline__ void coop_factor(LUT* __restrict__ a) {
     if (c == 0) lds_rd[i] = inv;
     if (pv != (T)0) {
       const bool mine = c > i && c < N;
+      // this lane's row multiplier (lane = row), handed to the column lanes
+      // by lane_value as each row's turn comes
+      const double fk = mine ? (double)a[c * MDX_LD + i] * inv : 0.0;
       const double ui = mine ? (double)a[i * MDX_LD + c] : 0.0;
 #pragma unroll 4
       for (int k = i + 1; k < N; ++k) {
-        const double f = (double)a[k * MDX_LD + i] * inv;
+        const double f = lane_value(fk, k);
         if (mine) a[k * MDX_LD + c] = (T)fmaT(-f, ui, (double)a[k * MDX_LD + c]);
       }
       __syncthreads();
-      if (mine) a[c * MDX_LD + i] = (T)((double)a[c * MDX_LD + i] * inv);
+      if (mine) a[c * MDX_LD + i] = (T)fk;
     }
     __syncthreads();
   }
+  // the row swaps composed into one permutation of the right-hand side,
+  // applied by coop_solve as a gather: position q takes r[lds_perm[q]]
+  if (c < N) lds_perm[c] = c;
+  __syncthreads();
+#pragma unroll 1
+  for (int i = 0; i < N; ++i) {
+    const int piv = lds_piv[i];
+    if (piv != i) {
+      const int x = lds_perm[i];
+      const int y = lds_perm[piv];
+      lds_perm[i] = y;
+      lds_perm[piv] = x;
+    }
+  }
+  __syncthreads();
 }
 
 // Solve with coop_factor's factorisation, the right-hand side in registers
-// (r, the same on every lane): the row swaps in order, the forward
+// (r, the same on every lane): the row swaps as the gather coop_factor
+// composed (the same values the swaps in order leave), the forward
 // substitution a row per lane (row c takes column j's term once row j is
 // final, so each row sums in ascending column order, as sequentially), then
 // the back substitution as one chain, every lane computing the same values.
@@ -661,18 +703,7 @@ __device__ __forceinline__ void coop_solve(const LUT* __restrict__ a, LUT* __res
 #pragma unroll
   for (int j = 0; j < N; ++j) lds_r[j] = r[j];
   __syncthreads();
-#pragma unroll 1
-  for (int i = 0; i < N; ++i) {
-    const int piv = lds_piv[i];
-    if (piv != i) {
-      const T x = lds_r[i];
-      const T y = lds_r[piv];
-      lds_r[i] = y;
-      lds_r[piv] = x;
-    }
-  }
-  __syncthreads();
-  T acc = c < N ? lds_r[c] : (T)0;
+  T acc = c < N ? lds_r[lds_perm[c]] : (T)0;
 #pragma unroll 1
   for (int j0 = 0; j0 < N - 1; j0 += MDX_CHUNK) {
     T l[MDX_CHUNK];
@@ -683,11 +714,12 @@ __device__ __forceinline__ void coop_solve(const LUT* __restrict__ a, LUT* __res
     for (int q = 0; q < MDX_CHUNK; ++q) {
       const int j = j0 + q;
       if (j < N - 1) {
-        const T rj = __shfl(acc, j);
+        const T rj = lane_value(acc, j);
         if (c > j && c < N) acc = fmaT(-l[q], rj, acc);
       }
     }
   }
+  __syncthreads();
   if (c < N) lds_r[c] = acc;
   __syncthreads();
 #pragma unroll 1
@@ -992,7 +1024,11 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
 #else
   LUT lu[MDX_NLU];
 #endif
-#if MDX_NEEDS_CLEAR
+#if MDX_COOP
+  // every slot to -0.0, a column per lane; the Jacobian then writes only its
+  // structurally nonzero slots (the others would have stored -0.0)
+  coop_clear(lu.base);
+#elif MDX_NEEDS_CLEAR
 #pragma unroll
   for (int q = 0; q < MDX_NLU; ++q) lu[q] = (LUT)0;
 #endif
