@@ -664,10 +664,17 @@ __device__ __forceinline__ void coop_factor(LUT* __restrict__ a) {
       // by lane_value as each row's turn comes
       const double fk = mine ? (double)a[c * MDX_LD + i] * inv : 0.0;
       const double ui = mine ? (double)a[i * MDX_LD + c] : 0.0;
-#pragma unroll 4
-      for (int k = i + 1; k < N; ++k) {
-        const double f = lane_value(fk, k);
-        if (mine) a[k * MDX_LD + c] = (T)fmaT(-f, ui, (double)a[k * MDX_LD + c]);
+      // eight rows at a time, their elements loaded before any is stored
+#pragma unroll 1
+      for (int k0 = i + 1; k0 < N; k0 += 8) {
+        T av[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) av[q] = mine ? a[clampN(k0 + q) * MDX_LD + c] : (T)0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const double f = lane_value(fk, clampN(k0 + q));
+          if (mine && k0 + q < N) a[(k0 + q) * MDX_LD + c] = (T)fmaT(-f, ui, (double)av[q]);
+        }
       }
       __syncthreads();
       if (mine) a[c * MDX_LD + i] = (T)fk;
@@ -719,26 +726,37 @@ __device__ __forceinline__ void coop_solve(const LUT* __restrict__ a, LUT* __res
       }
     }
   }
-  __syncthreads();
-  if (c < N) lds_r[c] = acc;
-  __syncthreads();
+  // back substitution: lane j keeps x_j once it is known, and row i reads
+  // each x_j it needs from lane j; the row's elements come from LDS
+  T xr = (T)0;
 #pragma unroll 1
   for (int i = N - 1; i >= 0; --i) {
-    T x = lds_r[i];
+    T x = lane_value(acc, i);
+    int j = i + 1;
+    // sixteen terms at a time, the row's sixteen elements loaded first
 #pragma unroll 1
-    for (int j0 = i + 1; j0 < N; j0 += MDX_CHUNK) {
-      T uv[MDX_CHUNK], rv[MDX_CHUNK];
+    for (; j + 16 <= N; j += 16) {
+      T u[16];
 #pragma unroll
-      for (int q = 0; q < MDX_CHUNK; ++q) {
-        uv[q] = a[i * MDX_LD + clampN(j0 + q)];
-        rv[q] = lds_r[clampN(j0 + q)];
-      }
+      for (int q = 0; q < 16; ++q) u[q] = a[i * MDX_LD + j + q];
 #pragma unroll
-      for (int q = 0; q < MDX_CHUNK; ++q)
-        if (j0 + q < N) x = fmaT(-uv[q], rv[q], x);
+      for (int q = 0; q < 16; ++q) x = fmaT(-u[q], lane_value(xr, j + q), x);
     }
-    lds_r[i] = x / a[i * MDX_LD + i];
+#pragma unroll 1
+    for (; j + 4 <= N; j += 4) {
+      T u[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) u[q] = a[i * MDX_LD + j + q];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) x = fmaT(-u[q], lane_value(xr, j + q), x);
+    }
+#pragma unroll 1
+    for (; j < N; ++j) x = fmaT(-a[i * MDX_LD + j], lane_value(xr, j), x);
+    x = x / a[i * MDX_LD + i];
+    if (c == i) xr = x;
   }
+  __syncthreads();
+  if (c < N) lds_r[c] = xr;
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < N; ++j) r[j] = lds_r[j];
