@@ -1204,7 +1204,9 @@ def augmented_trace(traced: Traced, direction_columns) -> Traced:
 def generate_sensitivity(traced: Traced):
     """Device source of ``sens_rhs``, ``sens_dt`` and ``sens_coupling`` (see
     ``SensitivityNodes``); each takes the state through an accessor, ``s``,
-    ``pd`` (and ``v``) through accessors, and writes ``out[0..n_vars)``."""
+    ``pd`` (and ``v``) through accessors, and writes ``out[0..n_vars)``
+    (for ``sens_coupling`` a pointer or an accessor whose elements take an
+    assignment of a double)."""
     sn = sensitivity_nodes(traced)
     g = traced.graph
     parts = []
@@ -1212,11 +1214,13 @@ def generate_sensitivity(traced: Traced):
                                ("sens_coupling", sn.coupling, ", const VA& v")):
         lines, _ = Emitter(g).body(nodes)
         outs = [f"  out[{j}] = {value_of(x)};" for j, x in enumerate(nodes)]
-        tmpl = "template <class PA, class YA, class SA, class DA" + (", class VA>" if extra else ">")
+        tmpl = "template <class PA, class YA, class SA, class DA" + (", class VA, class OA>"
+                                                                    if extra else ">")
+        out_t = "OA out" if extra else "double* __restrict__ out"
         parts.append("\n".join(
             [tmpl,
              f"__device__ __forceinline__ void {name}(const YA& y, const double t, const PA& p,",
-             f"    const SA& s, const DA& pd{extra}, double* __restrict__ out) {{",
+             f"    const SA& s, const DA& pd{extra}, {out_t}) {{",
              "  (void)t;", "  (void)p;", "  (void)s;", "  (void)pd;"]
             + lines + outs + ["}"]))
     return "\n\n".join(parts) + "\n"

@@ -176,6 +176,36 @@ def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None
     return defs
 
 
+def rodas5p_block(requested, n_aug: int, lu_precision: str, n_traj: int, n_cu: int,
+                  placement=None, sparsity=None) -> int:
+    """The workgroup size of a plain or joint Rodas5P solve: the caller's, else
+    a wavefront, except for a small ensemble of a sparse system whose rolled
+    step (``n_aug``, the joint vector's length, above ``UNROLLED_MAX_VARS``;
+    LDS placement) cannot keep its eight stage vectors in LDS at 64 lanes:
+    then the larger of 32 and 16 lanes at which the stage vectors, the stage
+    argument and the state all fit, provided every workgroup still has a SIMD
+    of its own (at most 4 per compute unit). One wave per SIMD either way, so
+    the narrower workgroup only moves the vectors out of scratch. The
+    reference's stiff Van der Pol rings, 1,000 trajectories, FP32 sparse LU:
+    the 128-variable ring 5.26 -> 3.61 ms, the joint solve of the 64-variable
+    ring with one parameter direction 7.82 -> 5.48 ms; the 64-variable ring's
+    plain solve (stage vectors in LDS at 64 lanes) keeps 64 lanes: 1.31 ms
+    there, 1.41-1.45 ms at 32 or 16."""
+    if requested is not None:
+        return trajectories_per_block_or_default(requested)
+    if placement not in (None, "lds") or sparsity is None or n_aug <= UNROLLED_MAX_VARS:
+        return DEFAULT_BLOCK
+    lu_bytes = 4 if lu_precision == "fp32" else 8
+    stage_vectors = 8 * n_aug * lu_bytes
+    if stage_vectors * DEFAULT_BLOCK <= LDS_BYTES_PER_CU:
+        return DEFAULT_BLOCK
+    for block in (32, 16):
+        if ((stage_vectors + 16 * n_aug) * block <= LDS_BYTES_PER_CU
+                and -(-n_traj // block) <= 4 * n_cu):
+            return block
+    return DEFAULT_BLOCK
+
+
 def tsit5_block(requested, n_vars: int) -> int:
     """The workgroup size of a Tsit5 solve: the caller's, else a wavefront,
     except that a rolled step (more than ``UNROLLED_MAX_VARS`` variables)

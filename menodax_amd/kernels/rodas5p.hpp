@@ -830,31 +830,55 @@ __device__ __forceinline__ void sens_time_derivative(const Z& y, double t, const
 #endif
 }
 
+// Stage S's state rows as doubles, read where the coupling term uses them.
+template <class KS>
+struct StageRows {
+  KS ks;
+  int S;
+  __device__ __forceinline__ double operator[](int j) const { return (double)ks(S, j); }
+};
+
+// A direction's right-hand side as the coupling term is produced: element j of
+// the term c lands in r[j] = (LUT)(sv[j] + c), so neither c nor a copy of the
+// stage's state rows is held whole next to r.
+struct CoupledRhs {
+  LUT* r;
+  const double* sv;
+  struct Elem {
+    LUT* r;
+    double s;
+    __device__ __forceinline__ void operator=(double c) const { *r = (LUT)(s + c); }
+  };
+  __device__ __forceinline__ Elem operator[](int j) const { return Elem{r + j, sv[j]}; }
+};
+
 // The sensitivity rows of stage S, after its state rows are solved: the joint
 // iteration matrix is block lower triangular with M on every diagonal block, so
 // each direction solves against the same factorisation with its right-hand side
 // corrected by L_d k_y, one second-order sweep at the step's base point
 // (rodas5P.py:512-561). ``sv`` holds the directions' stage values (double),
 // ``ks`` stores into the stage vectors.
+#if MDX_NSENS > 0
+template <class Z, class KS, class L>
+__device__ __forceinline__ void sens_direction(int S, int d, const Z& ybase, double t,
+                                               const PA& p, const double* __restrict__ sv,
+                                               const L& lu, const Pivots& ipiv, KS ks) {
+  LUT r[N];
+  sys::sens_coupling(ybase, t, p, Shifted<Z>{ybase, N + d * N}, OneHot{PCOL[d]},
+                     StageRows<KS>{ks, S}, CoupledRhs{r, sv});
+  solve(lu, ipiv, r);
+#pragma unroll
+  for (int j = 0; j < N; ++j) ks(S, N + d * N + j) = r[j];
+}
+#endif
+
 template <class Z, class KS, class L>
 __device__ __forceinline__ void sens_stage(int S, const Z& ybase, double t, const PA& p,
                                            const double* __restrict__ sv, const L& lu,
                                            const Pivots& ipiv, KS ks) {
 #if MDX_NSENS > 0
-  double ky[N];
-#pragma unroll
-  for (int j = 0; j < N; ++j) ky[j] = (double)ks(S, j);
 #pragma unroll MDX_SENS_UNROLL
-  for (int d = 0; d < NS; ++d) {
-    double c[N];
-    sys::sens_coupling(ybase, t, p, Shifted<Z>{ybase, N + d * N}, OneHot{PCOL[d]}, ky, c);
-    LUT r[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) r[j] = (LUT)(sv[d * N + j] + c[j]);
-    solve(lu, ipiv, r);
-#pragma unroll
-    for (int j = 0; j < N; ++j) ks(S, N + d * N + j) = r[j];
-  }
+  for (int d = 0; d < NS; ++d) sens_direction(S, d, ybase, t, p, sv + d * N, lu, ipiv, ks);
 #else
   (void)S, (void)ybase, (void)t, (void)p, (void)sv, (void)lu, (void)ipiv, (void)ks;
 #endif
@@ -993,28 +1017,103 @@ __device__ __forceinline__ bool active(const State& s, const SolveArgs& a) {
   return s.save_idx < a.n_save && s.t < s.tf && s.n_steps < a.max_steps;
 }
 
-// acc[jj] = sum_{q<S} c_q k_q[j0+jj] over one chunk of elements, q ascending
-// (the rolled step's stage sums). A chunk's loads are independent, so they are
+// acc[jj] = sum_{q<S} c_q k_q[j0+jj] for the first ``count`` elements of one
+// chunk, q ascending (the rolled step's stage sums). A chunk's loads are independent, so they are
 // all in flight before the first is waited for; summing every element at once
 // would keep one accumulator per element live and leave the loads no registers.
 template <class KS>
 __device__ __forceinline__ void chunk_sum(const double* __restrict__ c, int S, int j0, KS ks,
-                                          double* __restrict__ acc) {
+                                          double* __restrict__ acc, int count) {
 #pragma unroll
   for (int jj = 0; jj < MDX_CHUNK; ++jj)
-    if (j0 + jj < NA) acc[jj] = c[0] * (double)ks(0, j0 + jj);
+    if (jj < count) acc[jj] = c[0] * (double)ks(0, j0 + jj);
 #pragma unroll 1
   for (int q = 1; q < S; ++q) {
     const double cq = c[q];
     LUT kv[MDX_CHUNK];
 #pragma unroll
     for (int jj = 0; jj < MDX_CHUNK; ++jj)
-      if (j0 + jj < NA) kv[jj] = ks(q, j0 + jj);
+      if (jj < count) kv[jj] = ks(q, j0 + jj);
 #pragma unroll
     for (int jj = 0; jj < MDX_CHUNK; ++jj)
-      if (j0 + jj < NA) acc[jj] = fmaT(cq, (double)kv[jj], acc[jj]);
+      if (jj < count) acc[jj] = fmaT(cq, (double)kv[jj], acc[jj]);
   }
 }
+
+#if MDX_ROLLED
+// Rows [j_base, j_base + N) of stage S's right-hand side, rhs_j = work_j +
+// dt*d_S*dT_j + (sum_q C_Sq k_qj) / dt (rodas5P.py:611-647), with work holding
+// those rows' f values; the sum is formed a chunk at a time (see chunk_sum),
+// one branch per stage and chunk outside the element loop, and each row handed
+// to out(row - j_base, value).
+template <class KST, class OUT>
+__device__ __forceinline__ void stage_rows(int S, int j_base, const double* __restrict__ work,
+                                           const double* __restrict__ dT, KST kst,
+                                           double dt_use, double inv_dt, OUT out) {
+  (void)dT;
+  const double* __restrict__ cm = CM[S];
+#pragma unroll
+  for (int i0 = 0; i0 < N; i0 += MDX_CHUNK) {
+    const int j0 = j_base + i0;
+    if (S == 0) {
+      const double dd = dt_use * DV[0];
+#pragma unroll
+      for (int jj = 0; jj < MDX_CHUNK; ++jj) {
+        if (i0 + jj >= N) break;
+        out(i0 + jj, fmaT(dd, DTJ(j0 + jj), work[i0 + jj]));
+      }
+    } else {
+      double acc[MDX_CHUNK];
+      chunk_sum(cm, S, j0, kst, acc, N - i0);
+      if (S < 5) {
+        const double dd = dt_use * DV[S];
+#pragma unroll
+        for (int jj = 0; jj < MDX_CHUNK; ++jj) {
+          if (i0 + jj >= N) break;
+          out(i0 + jj, fmaT(acc[jj], inv_dt, fmaT(dd, DTJ(j0 + jj), work[i0 + jj])));
+        }
+      } else {
+#pragma unroll
+        for (int jj = 0; jj < MDX_CHUNK; ++jj) {
+          if (i0 + jj >= N) break;
+          out(i0 + jj, fmaT(acc[jj], inv_dt, work[i0 + jj]));
+        }
+      }
+    }
+  }
+}
+
+// Stage S of the rolled step at the stage argument z: the state rows' f,
+// right-hand side and solve, then each sensitivity direction's rows the same
+// way (sens_direction), so that one block of n_vars rows is held at a time
+// rather than the whole joint vector.
+template <class Z, class YB, class KS, class KST, class L>
+__device__ __forceinline__ void rolled_stage(int S, const Z& z, double ts, const YB& ybase,
+                                             double t, const PA& p,
+                                             const double* __restrict__ dT, KS ks, KST kst,
+                                             const L& lu, const Pivots& ipiv, double dt_use,
+                                             double inv_dt) {
+  (void)ybase, (void)t;
+  double work[N];
+  sys::rhs(z, ts, p, work);
+  LUT r[N];
+  stage_rows(S, 0, work, dT, kst, dt_use, inv_dt,
+             [&](int i, double v) { r[i] = (LUT)v; });
+  solve(lu, ipiv, r);
+#pragma unroll
+  for (int j = 0; j < N; ++j) ks(S, j) = r[j];
+#if MDX_NSENS > 0
+#pragma unroll MDX_SENS_UNROLL
+  for (int d = 0; d < NS; ++d) {
+    sys::sens_rhs(z, ts, p, Shifted<Z>{z, N + d * N}, OneHot{PCOL[d]}, work);
+    double sv[N];
+    stage_rows(S, N + d * N, work, dT, kst, dt_use, inv_dt,
+               [&](int i, double v) { sv[i] = v; });
+    sens_direction(S, d, ybase, t, p, sv, lu, ipiv, ks);
+  }
+#endif
+}
+#endif
 
 // One attempted step, accepted or rejected, with the saves it passes.
 __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
@@ -1118,7 +1217,7 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
 #pragma unroll
       for (int j0 = 0; j0 < NA; j0 += MDX_CHUNK) {
         double acc[MDX_CHUNK], yv[MDX_CHUNK];
-        chunk_sum(am, S, j0, kst, acc);
+        chunk_sum(am, S, j0, kst, acc, NA - j0);
 #pragma unroll
         for (int jj = 0; jj < MDX_CHUNK; ++jj)
           if (j0 + jj < NA) yv[jj] = YJ(s, j0 + jj);
@@ -1141,60 +1240,16 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
       }
       ts = t_end;
     }
-    double work[NA];
 #if MDX_LDS_U && MDX_LDS_Y
     // the first stage evaluates at the state itself, read where it lies
-    rhs_aug(LdsVec{S == 0 ? &lds_y[lane_] : &lds_u[lane_]}, ts, p, work);
+    rolled_stage(S, LdsVec{S == 0 ? &lds_y[lane_] : &lds_u[lane_]}, ts, ybase, t, p, dT, ks,
+                 kst, lu, ipiv, dt_use, inv_dt);
 #elif MDX_LDS_U
-    rhs_aug(LdsVec{&lds_u[lane_]}, ts, p, work);
+    rolled_stage(S, LdsVec{&lds_u[lane_]}, ts, ybase, t, p, dT, ks, kst, lu, ipiv, dt_use,
+                 inv_dt);
 #else
-    rhs_aug(u, ts, p, work);
+    rolled_stage(S, u, ts, ybase, t, p, dT, ks, kst, lu, ipiv, dt_use, inv_dt);
 #endif
-    // rhs_j = LU(work_j + dt*d_S*dT_j + (sum_q C_Sq k_qj) / dt) (rodas5P.py:611-647),
-    // the sum formed after the right-hand side, a chunk at a time, so that it
-    // never holds a register for every element; one branch per stage and
-    // chunk, outside the element loop, so each element is one formula rather
-    // than all three and a select
-    LUT r[N];
-    const double* __restrict__ cm = CM[S];
-#pragma unroll
-    for (int j0 = 0; j0 < NA; j0 += MDX_CHUNK) {
-      if (S == 0) {
-        const double dd = dt_use * DV[0];
-#pragma unroll
-        for (int jj = 0; jj < MDX_CHUNK; ++jj) {
-          const int j = j0 + jj;
-          if (j >= NA) break;
-          const double v = fmaT(dd, DTJ(j), work[j]);
-          if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
-        }
-      } else {
-        double acc[MDX_CHUNK];
-        chunk_sum(cm, S, j0, kst, acc);
-        if (S < 5) {
-          const double dd = dt_use * DV[S];
-#pragma unroll
-          for (int jj = 0; jj < MDX_CHUNK; ++jj) {
-            const int j = j0 + jj;
-            if (j >= NA) break;
-            const double v = fmaT(acc[jj], inv_dt, fmaT(dd, DTJ(j), work[j]));
-            if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
-          }
-        } else {
-#pragma unroll
-          for (int jj = 0; jj < MDX_CHUNK; ++jj) {
-            const int j = j0 + jj;
-            if (j >= NA) break;
-            const double v = fmaT(acc[jj], inv_dt, work[j]);
-            if (j < N) r[j] = (LUT)v; else sv[j - N] = v;
-          }
-        }
-      }
-    }
-    solve(lu, ipiv, r);
-#pragma unroll
-    for (int j = 0; j < N; ++j) KQ(S, j) = r[j];
-    sens_stage(S, ybase, t, p, sv, lu, ipiv, ks);
   }
 #undef MDX_PIN
 #else

@@ -235,6 +235,25 @@ def test_a_rolled_tsit5_step_takes_the_largest_block_its_stage_vectors_fit():
     assert _kernels.tsit5_placement(64, 64)["MDX_LDS_K"] == 0
 
 
+def test_a_small_sparse_ensemble_narrows_its_workgroup_to_keep_stage_vectors_in_lds():
+    B = _kernels.rodas5p_block
+    ring = ((0,),)
+    # 128 joint rows, FP32: 4 KB of stage vectors a lane, not at 64 lanes
+    assert B(None, 128, "fp32", 1000, 256, None, ring) == 16
+    assert B(None, 128, "fp64", 1000, 256, None, ring) == 16
+    assert B(None, 64, "fp64", 1000, 256, None, ring) == 32
+    # stage vectors fit at 64 lanes; dense; a placement chosen; the caller's size
+    assert B(None, 64, "fp32", 1000, 256, None, ring) == 64
+    assert B(None, 128, "fp32", 1000, 256, None, None) == 64
+    assert B(None, 128, "fp32", 1000, 256, "rolled", ring) == 64
+    assert B(8, 128, "fp32", 1000, 256, None, ring) == 8
+    # more workgroups than SIMDs, or nothing fits at 16 lanes: a wavefront
+    assert B(None, 128, "fp32", 20000, 256, None, ring) == 64
+    assert B(None, 440, "fp32", 1000, 256, None, ring) == 64
+    place = _kernels.rodas5p_placement(128, "fp32", 16, "lds")
+    assert place["MDX_LDS_K"] == place["MDX_LDS_U"] == place["MDX_LDS_Y"] == 1
+
+
 def test_the_wave_placement_is_for_small_ensembles_of_dense_systems():
     W = _kernels.wave_placement
     assert W(64, 1000, 256) == "wave" and W(24, 1024, 256) == "wave"

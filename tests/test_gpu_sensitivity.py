@@ -179,3 +179,26 @@ def test_stats_stay_available_under_grad():
     assert hist.requires_grad and not stats["accepted_steps"].requires_grad
     (hist ** 2).sum().backward()
     assert p.grad is not None and torch.isfinite(p.grad).all()
+
+
+def test_a_small_sparse_joint_solve_runs_narrow_workgroups_with_the_same_result():
+    """1,000 64-variable stiff rings with one parameter direction: 128 joint
+    rows, whose stage vectors fit in LDS at 16 lanes, not at 64; the size rule's
+    workgroup gives the 64-lane kernel's outputs bit for bit."""
+    import torch
+
+    from menodax_amd import _kernels
+
+    dev = torch.device("cuda", 0)
+    fn, _ = S.vdp_lattice(32)
+    mask = S.vdp_sparsity(32)
+    y0n, pn = S.vdp_scenario(32, 1000, divergence=0.5)
+    y0, p = torch.tensor(y0n, device=dev), torch.tensor(pn, device=dev)
+    t = torch.tensor(S.VDP_TIMES, device=dev)
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    assert _kernels.rodas5p_block(None, 128, "fp32", 1000, n_cu, None, mask) == 16
+    kw = dict(first_step=1e-4, rtol=1e-6, atol=1e-8, sparsity=mask, wrt_y0=False)
+    hist, sens = rodas5P.solve_sensitivities(fn, y0, t, p, **kw)
+    hist64, sens64 = rodas5P.solve_sensitivities(fn, y0, t, p, trajectories_per_block=64, **kw)
+    assert torch.equal(hist, hist64) and torch.equal(sens, sens64)
+    assert torch.isfinite(sens).all()
