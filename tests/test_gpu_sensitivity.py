@@ -202,3 +202,13 @@ def test_a_small_sparse_joint_solve_runs_narrow_workgroups_with_the_same_result(
     hist64, sens64 = rodas5P.solve_sensitivities(fn, y0, t, p, trajectories_per_block=64, **kw)
     assert torch.equal(hist, hist64) and torch.equal(sens, sens64)
     assert torch.isfinite(sens).all()
+
+    # the same joint kernels under autograd: a gradient of a plain solve
+    def grad(block):
+        q = p.clone().requires_grad_(True)
+        h = rodas5P.solve(fn, y0, t, q, first_step=1e-4, rtol=1e-6, atol=1e-8, sparsity=mask,
+                          trajectories_per_block=block)
+        h[:, -1].sum().backward()
+        return q.grad
+
+    assert torch.equal(grad(None), grad(64))
