@@ -142,7 +142,8 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64, variants=UNEVEN_VAR
     ``variants`` narrows that to some of them (profiling runs), which leaves
     out the comparison and the endpoint error unless both headline variants
     are among them."""
-    from menodax_amd import _kernels, _sparse_direct as SD, rodas5P, roofline, systems as S
+    from menodax_amd import _kernels, _sparse_direct as SD, rodas5P, roofline
+    from tests import systems as S
     from menodax_amd._sparsity import normalize_sparsity
 
     n_vars = 50
@@ -251,7 +252,7 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64, variants=UNEVEN_VAR
     # tolerance, and a tight FP64 solve
     idx = np.arange(sample)
     system = O.linquad(n_vars, rows, cols, vals, 0.01, off_pidx=pidx, n_p=n_p)
-    layout = SD.oracle_tables(sp, "amd")
+    layout = O.sparse_structure(sp, SD.fill_reducing_order(sp, "amd"))
     same_tol, _ = O.rodas5p(system, y0[idx], times, p[idx], layout=layout, rtol=RTOL, atol=ATOL,
                             lu_precision="fp64")
     tight, _ = O.rodas5p(system, y0[idx], times, p[idx], layout=layout, rtol=1e-10, atol=1e-12,
@@ -284,7 +285,8 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, sample=64, variants=UNEVEN_VAR
 def lorenz_block(torch, dev, n=1024):
     """C1: Tsit5 on the Lorenz ensemble -- rho ~ U[0, 28] (seed 0), y0 = (1, 0, 0),
     t in [0, 1] with 100 saves, rtol 1e-6 atol 1e-8."""
-    from menodax_amd import _kernels, rodas5P, systems as S
+    from menodax_amd import _kernels, rodas5P
+    from tests import systems as S
 
     y0, p = S.lorenz_rho_ensemble(n)
     t = torch.tensor(np.linspace(0.0, 1.0, 100), device=dev)
@@ -324,7 +326,8 @@ def vdp_chain_block(torch, dev, log_fn, n=16384):
     """C3: Rodas5P, sparse FP32 LU (AMD) on 32 diffusively coupled Van der Pol
     oscillators (64 vars, open chain), mu ~ LogUniform[10, 1000], y0 ~ U[-2, 2],
     t in [0, 2000] with 11 saves, rtol 1e-6 atol 1e-8, max_steps 100,000."""
-    from menodax_amd import _kernels, rodas5P, roofline, systems as S
+    from menodax_amd import _kernels, rodas5P, roofline
+    from tests import systems as S
     from menodax_amd._sparsity import normalize_sparsity
 
     fn, _ = S.vdp_lattice(32, mu=1.0, d=0.1, wrap=False)
@@ -366,7 +369,7 @@ def vdp_chain_block(torch, dev, log_fn, n=16384):
     idx = np.argsort(att)[:8]
     system = O.vdp_lattice(32, mu=1.0, d=0.1, wrap=False)
     same, st = O.rodas5p(system, y0[idx], np.linspace(0.0, 2000.0, 11), mu[idx],
-                         layout=SD.oracle_tables(mask, "amd"), rtol=RTOL, atol=ATOL,
+                         layout=O.sparse_structure(mask, SD.fill_reducing_order(mask, "amd")), rtol=RTOL, atol=ATOL,
                          lu_precision="fp32")
     hist = o.ensemble_view()[idx].cpu().numpy()
     res["endpoint_error"] = {
@@ -380,7 +383,8 @@ def vdp_chain_block(torch, dev, log_fn, n=16384):
 
 def adjoint_block(torch, log_fn, n=8192, ckpt_every=64):
     """C4: d(loss)/d(mu, y0) for stiff Van der Pol ensembles."""
-    from menodax_amd import rodas5P, systems as S
+    from menodax_amd import rodas5P
+    from tests import systems as S
 
     out = {"workload": f"{n} stiff Van der Pol trajectories, mu ~ LogUniform[10, 100], "
                        "y0 ~ U[-2, 2], t in [0, 3], 20 saves, loss = sum of squared errors vs a "
@@ -454,7 +458,8 @@ def tsit5_adjoint_line(torch, log_fn, n, ckpt_every):
     """The Tsit5 discrete adjoint on a Lorenz ensemble of the same size:
     rho ~ U[5, 28], y0 near (1, 0, 0), t in [0, 1], 20 saves, the loss against a
     target solved at 1.05 rho plus noise, rtol 1e-8 atol 1e-10."""
-    from menodax_amd import systems as S, tsit5
+    from menodax_amd import tsit5
+    from tests import systems as S
 
     rng = np.random.default_rng(0)
     rho = rng.uniform(5.0, 28.0, size=(n, 1))
@@ -531,7 +536,8 @@ def reference_benchmarks(torch, log_fn, reps=5):
     forward sensitivities (the reference's jax.value_and_grad), sparse and
     dense. rtol 1e-6, atol 1e-8 throughout. The dense rings are also run with
     the autotuner's choice (reported next to the default's time)."""
-    from menodax_amd import autotune, rodas5P, systems as S, tsit5
+    from menodax_amd import autotune, rodas5P, tsit5
+    from tests import systems as S
 
     dev = torch.device("cuda", 0)
 
@@ -601,7 +607,8 @@ def wave_ring(torch, log_fn, reps=3):
     """The reference's 64-variable stiff Van der Pol ring, 1,000 identical
     trajectories, dense LU: the per-lane kernel (placement "lds") against the
     wave placement, FP32 and FP64 LU, one warm-up and ``reps`` timed solves."""
-    from menodax_amd import rodas5P, systems as S
+    from menodax_amd import rodas5P
+    from tests import systems as S
 
     dev = torch.device("cuda", 0)
     fn, _ = S.vdp_lattice(32)
@@ -677,7 +684,8 @@ def main():
 
     import torch
 
-    from menodax_amd import _kernels, _runtime, roofline, rodas5P, systems as S
+    from menodax_amd import _kernels, _runtime, roofline, rodas5P
+    from tests import systems as S
 
     if args.only_wave_ring:
         print(json.dumps({"wave_ring": wave_ring(torch, log)}), flush=True)

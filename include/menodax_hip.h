@@ -8,8 +8,10 @@
  * that function, a stream, the launch shape and the solver's buffers
  * (LaunchNumbaCudaAbi, :145-213), and the XLA FFI symbol that exposes the
  * launcher (modax_numba_cuda_abi_launch, :215-231). The entry points below
- * replace them one for one; the XLA handler built on them lives in
- * menodax_amd/csrc/xla_ffi.cpp.
+ * replace them one for one. An XLA FFI handler is a thin wrapper over
+ * mdx_solve_launch (its binding is sketched in INTEGRATION.md); it is not
+ * shipped, because no jaxlib with the FFI headers exists on the machines this
+ * library is built and tested on.
  *
  * Conventions: every function returns 0 on success and a nonzero code on
  * failure, with the message available from mdx_last_error() on the calling

@@ -1,38 +1,45 @@
 """A sparse direct solver compiled for one sparsity pattern.
 
-Restates the host analysis of modax/_sparse_direct.py and emits HIP device
-code in place of its numba device functions:
+The reference's interface (modax/_sparse_direct.py) is kept -- ``ORDERINGS``,
+``fill_reducing_order`` (:91-152), ``fill_pattern`` (:155-184), ``analyse``
+and ``SparseLULayout`` (:187-349), ``compressed_jacobian`` (:352-390), the
+solver object and its cached constructors (:393-452), the unrolling
+thresholds (:468-469) -- over this module's own construction:
 
-  1. order   -- a symmetric fill-reducing permutation (``_amd.amd_order``) or
-                the natural order (:91-152); the diagonal stays on the
-                diagonal, where I/(h gamma) guarantees a structural pivot;
-  2. fill    -- the exact L+U pattern of an unpivoted LU of the permuted
-                pattern, by Gaussian elimination on the pattern alone
-                (:155-184);
-  3. layout  -- one CSR image of L+U in elimination order; row/column origins
-                translate to the caller's indices so no permutation pass is
-                needed (:187-297, 307-349);
-  4. compile -- the up-looking factorisation (inverse diagonal stored) and the
-                two triangular sweeps (:472-622), emitted straight-line with
-                every slot a literal when small enough and table-driven from
-                constant memory otherwise, at the reference's thresholds.
+  1. order    -- a symmetric fill-reducing permutation (``_amd.amd_order``) or
+                 the natural order; the diagonal stays on the diagonal, where
+                 I/(h gamma) guarantees a structural pivot;
+  2. symbolic -- one up-looking pass over the permuted rows: row i of L+U is
+                 row i of the pattern plus, for each column k < i that it
+                 holds or gains on the way (taken in increasing order from a
+                 heap), the columns right of the diagonal of the finished row
+                 k. The same pass records, for every multiplier slot, the slots
+                 of row i its update writes, so no later search is needed;
+  3. layout   -- the rows concatenated in elimination order as one CSR image,
+                 row and column origins translating back to the caller's
+                 indices so the device code never permutes a vector;
+  4. compile  -- the factorisation (inverse diagonal stored) and the two
+                 triangular sweeps as HIP, straight-line with literal slots
+                 below the thresholds and loops over ``__constant__`` index
+                 tables above them.
 
-The colouring of the caller's pattern and the slot map that points each
-coloured Jacobian entry at its CSR slot are ``compressed_jacobian`` (:352-390).
 There is no pivoting: the same pattern serves every trajectory, and Rodas5P
-being a W-method, an inexact factorisation costs step-size control rather
-than order (the controller rejects a step whose factorisation went bad).
+being a W-method, an inexact factorisation costs step-size control rather than
+order (the controller rejects a step whose factorisation went bad).
 """
 
 from __future__ import annotations
 
+import bisect
 import functools
-from dataclasses import dataclass, replace
+import heapq
+import itertools
+from dataclasses import replace
 
 import numpy as np
 
 from ._amd import amd_order
-from ._sparsity import CompressedJacobian, colour_sparsity, normalize_sparsity
+from ._sparsity import CompressedJacobian, SlotStorage, colour_sparsity, normalize_sparsity
 
 ORDERINGS = ("amd", "natural")
 
@@ -43,40 +50,89 @@ MAX_UNROLLED_UPDATES = 8192
 
 
 def fill_reducing_order(pattern, ordering: str = "amd") -> tuple:
+    """Elimination order as caller indices: position i is eliminated i-th."""
     if ordering not in ORDERINGS:
         raise ValueError(f"unknown ordering {ordering!r}; expected one of {ORDERINGS}")
-    n = len(pattern)
     if ordering == "natural":
-        return tuple(range(n))
+        return tuple(range(len(pattern)))
     return amd_order(pattern)
 
 
+def _symbolic_rows(rows):
+    """Up-looking symbolic LU without pivoting.
+
+    ``rows[i]`` holds the columns of row i (elimination indices). Returns each
+    row's sorted L+U columns, diagonal included. Row i's lower columns are
+    visited in increasing order; visiting k adds the finished row k's columns
+    right of k, and any of those left of i is visited in turn. This is the
+    least pattern closed under elimination -- what an unpivoted factorisation
+    writes, and what elimination on the boolean pattern produces."""
+    finished = []
+    right_of_diagonal = []
+    for i, given in enumerate(rows):
+        members = set(given)
+        members.add(i)
+        pending = [k for k in members if k < i]
+        heapq.heapify(pending)
+        while pending:
+            k = heapq.heappop(pending)
+            for j in right_of_diagonal[k]:
+                if j not in members:
+                    members.add(j)
+                    if j < i:
+                        heapq.heappush(pending, j)
+        cols = sorted(members)
+        finished.append(cols)
+        right_of_diagonal.append(cols[bisect.bisect_right(cols, i):])
+    return finished
+
+
 def fill_pattern(pattern) -> np.ndarray:
-    """The exact L+U pattern of an unpivoted LU (diagonal forced present):
-    eliminating column k, every row below with an entry there inherits row
-    k's entries right of the diagonal."""
+    """The exact L+U pattern of an unpivoted LU of ``pattern`` in its own
+    order (diagonal included), as an (n, n) boolean matrix."""
     n = len(pattern)
     filled = np.zeros((n, n), dtype=bool)
-    for row, cols in enumerate(pattern):
-        filled[row, list(cols)] = True
-    np.fill_diagonal(filled, True)
-    for k in range(n):
-        below = np.flatnonzero(filled[k + 1 :, k]) + k + 1
-        if below.size:
-            filled[below, k + 1 :] |= filled[k, k + 1 :]
+    for i, cols in enumerate(_symbolic_rows(pattern)):
+        filled[i, cols] = True
     return filled
 
 
-@dataclass(frozen=True)
 class SparseLULayout:
-    """CSR image of L+U in elimination order: row i is caller variable
-    ``order[i]``; L strictly left of ``diag_ptr``, U from it rightwards."""
+    """L+U of one pattern as a CSR image in elimination order.
 
-    n_vars: int
-    order: tuple
-    row_ptr: np.ndarray
-    col_ind: np.ndarray
-    diag_ptr: np.ndarray
+    Row i of the image is caller variable ``order[i]``. In each row the slots
+    left of ``diag_ptr[i]`` hold L (unit diagonal implied), ``diag_ptr[i]``
+    the diagonal (its inverse, once factorised) and the rest U."""
+
+    def __init__(self, order, rows):
+        self.n_vars = len(rows)
+        self.order = tuple(int(v) for v in order)
+        lengths = np.fromiter((len(r) for r in rows), dtype=np.int64, count=self.n_vars)
+        self.row_ptr = np.zeros(self.n_vars + 1, dtype=np.int32)
+        self.row_ptr[1:] = np.cumsum(lengths)
+        self.col_ind = np.fromiter(itertools.chain.from_iterable(rows), dtype=np.int32,
+                                   count=int(lengths.sum()))
+        below = np.fromiter((bisect.bisect_left(r, i) for i, r in enumerate(rows)),
+                            dtype=np.int32, count=self.n_vars)
+        self.diag_ptr = (self.row_ptr[:-1] + below).astype(np.int32)
+        self._updates = self._update_program(rows)
+
+    def _update_program(self, rows):
+        """For each multiplier slot p = (i, k): the slots of row i that row
+        k's upper entries update, in row k's order (so the sources are read
+        contiguously from row k and only destinations need an index)."""
+        dst_start = np.zeros(self.nnz, dtype=np.int32)
+        destination = []
+        for i, cols in enumerate(rows):
+            base = int(self.row_ptr[i])
+            where = {c: base + j for j, c in enumerate(cols)}
+            for p in range(base, int(self.diag_ptr[i])):
+                k = int(self.col_ind[p])
+                dst_start[p] = len(destination)
+                upper_k = rows[k][int(self.diag_ptr[k] - self.row_ptr[k]) + 1:]
+                destination.extend(where[c] for c in upper_k)
+        upper_end = self.row_ptr[1:][self.col_ind].astype(np.int32)
+        return dst_start, np.asarray(destination, dtype=np.int32), upper_end
 
     @property
     def nnz(self) -> int:
@@ -84,93 +140,65 @@ class SparseLULayout:
 
     @property
     def row_origin(self) -> np.ndarray:
+        """Caller index of each elimination row."""
         return np.asarray(self.order, dtype=np.int32)
 
     @property
     def col_origin(self) -> np.ndarray:
-        return np.asarray(self.order, dtype=np.int32)[self.col_ind]
+        """Caller index of each slot's column."""
+        return self.row_origin[self.col_ind]
 
     def slot_table(self) -> np.ndarray:
         """(n, n) caller (row, col) -> slot, -1 where the structure has nothing."""
         table = np.full((self.n_vars, self.n_vars), -1, dtype=np.int64)
-        origin = np.asarray(self.order, dtype=np.int64)
-        for i in range(self.n_vars):
-            a, b = int(self.row_ptr[i]), int(self.row_ptr[i + 1])
-            table[origin[i], origin[self.col_ind[a:b]]] = np.arange(a, b)
+        caller_row = np.repeat(self.row_origin, np.diff(self.row_ptr))
+        table[caller_row, self.col_origin] = np.arange(self.nnz)
         return table
 
     def factorization_ops(self):
-        """For each multiplier slot p = (i, k): where its updates start in the
-        destination list, the destination slots (in row k's upper order), and
-        the end of row k's upper part -- so the inner loop reads its sources
-        contiguously and only the destination needs an index."""
-        n = self.n_vars
-        dst_start = np.zeros(self.nnz, dtype=np.int64)
-        upper_end = np.asarray(self.row_ptr[1:], dtype=np.int64)[np.asarray(self.col_ind, np.int64)]
-        elim = np.full((n, n), -1, dtype=np.int64)
-        for i in range(n):
-            a, b = int(self.row_ptr[i]), int(self.row_ptr[i + 1])
-            elim[i, self.col_ind[a:b]] = np.arange(a, b)
-        dst = []
-        for i in range(n):
-            for p in range(int(self.row_ptr[i]), int(self.diag_ptr[i])):
-                k = int(self.col_ind[p])
-                dst_start[p] = len(dst)
-                for q in range(int(self.diag_ptr[k]) + 1, int(self.row_ptr[k + 1])):
-                    dst.append(int(elim[i, int(self.col_ind[q])]))
-        return dst_start.astype(np.int32), np.asarray(dst, dtype=np.int32), upper_end.astype(np.int32)
+        """``(dst_start, destination, upper_end)`` indexed by multiplier slot:
+        where its destinations start in ``destination``, and the end of the
+        pivot row's upper part (its sources run from the pivot's slot + 1)."""
+        return self._updates
 
     def flop_counts(self):
         """(multipliers, updates, substitutions): one factorisation's
         multipliers and multiply-add updates, one solve's multiply-adds."""
-        lower = self.diag_ptr - self.row_ptr[:-1]
-        upper = self.row_ptr[1:] - self.diag_ptr - 1
-        multipliers = int(lower.sum())
-        slot = np.arange(self.nnz)
-        row = np.repeat(np.arange(self.n_vars), np.diff(self.row_ptr))
-        lower_mask = slot < self.diag_ptr[row]
-        updates = int(upper[self.col_ind[lower_mask]].sum())
-        return multipliers, updates, multipliers + int(upper.sum())
+        lower = int((self.diag_ptr - self.row_ptr[:-1]).sum())
+        upper = int((self.row_ptr[1:] - self.diag_ptr - 1).sum())
+        return lower, int(self._updates[1].size), lower + upper
+
+    def __repr__(self):
+        return f"<SparseLULayout n_vars={self.n_vars} nnz={self.nnz} order={self.order}>"
 
 
 def analyse(pattern, ordering: str = "amd") -> SparseLULayout:
     """Order, factorise symbolically, lay out L+U in CSR."""
-    n = len(pattern)
     order = fill_reducing_order(pattern, ordering)
-    fwd = np.asarray(order, dtype=np.int64)
-    inv = np.empty(n, dtype=np.int64)
-    inv[fwd] = np.arange(n)
-    filled = fill_pattern(
-        tuple(tuple(sorted(int(inv[c]) for c in pattern[int(fwd[i])])) for i in range(n))
-    )
-    row_ptr = np.zeros(n + 1, dtype=np.int32)
-    row_ptr[1:] = np.cumsum(filled.sum(axis=1))
-    col_ind = np.concatenate([np.flatnonzero(filled[i]) for i in range(n)]).astype(np.int32)
-    diag_ptr = np.array(
-        [int(row_ptr[i]) + int(np.searchsorted(col_ind[row_ptr[i] : row_ptr[i + 1]], i))
-         for i in range(n)],
-        dtype=np.int32,
-    )
-    return SparseLULayout(n, tuple(int(i) for i in order), row_ptr, col_ind, diag_ptr)
+    position = np.empty(len(pattern), dtype=np.int64)
+    position[np.asarray(order, dtype=np.int64)] = np.arange(len(pattern))
+    permuted = [[int(position[c]) for c in pattern[v]] for v in order]
+    return SparseLULayout(order, _symbolic_rows(permuted))
 
 
 def compressed_jacobian(pattern, layout: SparseLULayout) -> CompressedJacobian:
     """The caller pattern's colouring, with each (row, colour) pointed at the
-    CSR slot of its entry; fill-in slots belong to no column and stay unwritten."""
-    compressed = colour_sparsity(pattern)
+    CSR slot of its entry; fill-in slots belong to no column and stay
+    unwritten by the sweeps."""
+    colouring = colour_sparsity(pattern)
+    n, width = layout.n_vars, colouring.n_colours
     table = layout.slot_table()
-    packed = np.full(layout.n_vars * compressed.n_colours, -1, dtype=np.int64)
-    for row, cols in enumerate(pattern):
-        for col in cols:
-            s = int(table[row, col])
-            if s < 0:
-                raise ValueError(f"entry ({row}, {col}) is in the pattern but not the structure")
-            packed[row * compressed.n_colours + compressed.colour[col]] = s
-    diagonal = tuple(int(table[i, i]) for i in range(layout.n_vars))
-    return replace(
-        compressed, packed=tuple(int(s) for s in packed), packed_diagonal=diagonal,
-        n_slots=layout.nnz,
-    )
+    rows = np.repeat(np.arange(n), [len(cols) for cols in pattern])
+    cols = np.fromiter(itertools.chain.from_iterable(pattern), dtype=np.int64, count=rows.size)
+    slots = table[rows, cols]
+    if (slots < 0).any():
+        k = int(np.argmax(slots < 0))
+        raise ValueError(f"entry ({rows[k]}, {cols[k]}) is in the pattern but not the structure")
+    grid = np.full(n * width, -1, dtype=np.int64)
+    grid[rows * width + np.asarray(colouring.colour, dtype=np.int64)[cols]] = slots
+    storage = SlotStorage(grid=tuple(int(s) for s in grid),
+                          diagonal=tuple(int(s) for s in np.diagonal(table)), size=layout.nnz)
+    return replace(colouring, storage=storage)
 
 
 class SparseDirectSolver:
@@ -214,19 +242,6 @@ def sparse_direct_solver_for(pattern, ordering: str = "amd") -> SparseDirectSolv
 
 def sparse_direct_solver(sparsity, n_vars: int, *, ordering: str = "amd") -> SparseDirectSolver:
     return sparse_direct_solver_for(normalize_sparsity(sparsity, n_vars), ordering)
-
-
-def oracle_tables(pattern, ordering: str = "amd"):
-    """The structure as the C oracle takes it: (nnz, row_ptr, col_ind,
-    diag_ptr, order, slot_of (n*n, caller indices), pattern mask (n*n))."""
-    layout = analyse(pattern, ordering)
-    n = layout.n_vars
-    mask = np.zeros((n, n), dtype=np.int32)
-    for r, cols in enumerate(pattern):
-        mask[r, list(cols)] = 1
-    return (layout.nnz, layout.row_ptr, layout.col_ind, layout.diag_ptr,
-            np.asarray(layout.order, np.int32), layout.slot_table().astype(np.int32).ravel(),
-            mask.ravel())
 
 
 # --- device source ------------------------------------------------------------

@@ -1,6 +1,7 @@
-"""Build the native parts in-tree: ``libmenodax_hip.so`` (the C ABI of
-``include/menodax_hip.h``) and, when a JAX install provides the XLA FFI
-headers, the XLA handler on top of it.
+"""Build the native part in-tree: ``libmenodax_hip.so``, the C ABI of
+``include/menodax_hip.h``. (An XLA FFI handler over that ABI would be built the
+same way against jaxlib's FFI headers; none is shipped, since no jaxlib exists
+where this library is built and tested.)
 
 The per-right-hand-side kernels are not built here: they are generated and
 compiled with hipRTC when a solve first needs them (``menodax_amd._runtime``).
@@ -17,7 +18,6 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmenodax_hip.so")
-XLA_LIB = os.path.join(HERE, "libmenodax_xla.so")
 
 # Every HIP translation unit is built with packed-FP32 instructions disabled.
 HIP_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -46,29 +46,8 @@ def build_runtime(verbose: bool = False) -> str:
     return LIB
 
 
-def build_xla_handler(verbose: bool = False) -> str | None:
-    """The XLA FFI handler needs jaxlib's ``xla/ffi/api/ffi.h``; skipped without it."""
-    try:
-        import jax.ffi  # noqa: F401
-
-        include = jax.ffi.include_dir()
-    except Exception:
-        return None
-    cmd = [
-        _hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", "-fPIC", "-shared", *HIP_FLAGS,
-        "-I", os.path.join(ROOT, "include"), "-I", include,
-        os.path.join(HERE, "csrc", "xla_ffi.cpp"), "-o", XLA_LIB,
-        "-L", HERE, "-lmenodax_hip", f"-Wl,-rpath,{HERE}",
-    ]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    return XLA_LIB
-
-
 def build_all(verbose: bool = False):
     build_runtime(verbose)
-    build_xla_handler(verbose)
 
 
 if __name__ == "__main__":

@@ -6,7 +6,7 @@ baseline, never as the product. ``menodax_amd`` does not import this module.
 
 Each ``System`` here names one of the oracle's hand-written right-hand sides
 (closed-form Jacobian included). The product integrates the matching Python
-``ode_fn`` from ``menodax_amd.systems``; the two descriptions are written
+``ode_fn`` from ``tests/systems.py``; the two descriptions are written
 independently, which is what makes a parity test between them mean something.
 
 Parity status of each oracle function (how it is pinned, see the tests named):
@@ -176,6 +176,50 @@ def kaps(n_pairs, epsilon_min):
         [10.0 ** (np.log10(epsilon_min) * i / (n_pairs - 1)) for i in range(n_pairs)]
     )
     return System("kaps", 2 * n_pairs, 1, inv_eps=1.0 / eps)
+
+
+def sparse_structure(pattern, order):
+    """The sparse structure ``rodas5p(..., layout=)`` takes, built here from the
+    pattern and an elimination order alone, independently of the product's
+    symbolic analysis. Restates the reference's:
+      - fill by Gaussian elimination on the boolean pattern, right-looking,
+        diagonal forced present (modax/_sparse_direct.py:155-184), on the
+        pattern permuted *before* the fill (:321-329);
+      - CSR of L+U in elimination order with ascending columns and the
+        diagonal's position per row (:330-349);
+      - the caller (row, col) -> slot table (:226-233).
+    ``pattern`` is a mask or rows of column indices; ``order[i]`` is the caller
+    variable eliminated i-th. Returns ``(nnz, row_ptr, col_ind, diag_ptr,
+    order, slot_of (n*n, caller indices), mask (n*n))``.
+
+    The order is an input: the AMD permutation itself is parity-unpinned (the
+    reference's SuiteSparse AMD is unavailable), so the tests hand the oracle
+    the product's permutation and pin the fill, layout and arithmetic here."""
+    if isinstance(pattern, np.ndarray) and pattern.ndim == 2 and pattern.dtype == bool:
+        mask = pattern.copy()
+    else:
+        n0 = len(pattern)
+        mask = np.zeros((n0, n0), dtype=bool)
+        for r, cols in enumerate(pattern):
+            mask[r, list(cols)] = True
+    n = mask.shape[0]
+    perm = np.asarray(order, dtype=np.int64)
+    if sorted(perm.tolist()) != list(range(n)):
+        raise ValueError("order is not a permutation")
+    work = mask[np.ix_(perm, perm)].copy()
+    work[np.arange(n), np.arange(n)] = True
+    for k in range(n):
+        rows = np.flatnonzero(work[k + 1:, k]) + k + 1
+        work[rows, k + 1:] |= work[k, k + 1:]
+    counts = work.sum(axis=1)
+    row_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    col_ind = np.nonzero(work)[1].astype(np.int32)  # row-major: ascending columns per row
+    diag_ptr = (row_ptr[:-1] + np.array([work[i, :i].sum() for i in range(n)])).astype(np.int32)
+    slot_of = np.full((n, n), -1, dtype=np.int32)
+    er, ec = np.nonzero(work)
+    slot_of[perm[er], perm[ec]] = np.arange(er.size, dtype=np.int32)
+    return (int(row_ptr[-1]), row_ptr, col_ind, diag_ptr, perm.astype(np.int32),
+            slot_of.ravel(), mask.astype(np.int32).ravel())
 
 
 def _ptr(a, ctype):

@@ -23,7 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
-from menodax_amd import systems as S  # noqa: E402
+from tests import systems as S  # noqa: E402
 from menodax_amd._sparsity import normalize_sparsity  # noqa: E402
 
 REFERENCE_SPARSITY = "/root/reference/modax/_sparsity.py"

@@ -478,3 +478,53 @@ def _check_divergence(divergence):
     if not np.isfinite(divergence) or divergence < 0.0:
         raise ValueError("divergence must be finite and non-negative")
     return float(divergence)
+
+
+# --------------------------------------------------------------------------
+# A bordered-block system whose sparse LU exceeds the unrolling thresholds
+# --------------------------------------------------------------------------
+def bordered_structure(core=45, tails=7, length=15, seed=5):
+    """A dense ``core`` x ``core`` block with ``tails`` tridiagonal chains of
+    ``length`` variables hanging off it (chain t's first variable coupled both
+    ways to core variable t % core): the shape of the reference's
+    Einstein-Boltzmann-like bordered patterns (tests/test_sparse_direct.py:
+    175-190), with a core large enough that one factorisation takes more than
+    8,192 multiply-add updates and one solve more than 2,048 substitutions, so
+    the solver is emitted table-driven without lowering any threshold.
+
+    The diagonal is per trajectory (p[0..n-1], negative); off-diagonal entries
+    are fixed, small enough that A stays diagonally dominant. Returns
+    ``(rows, cols, vals, mask)`` of the off-diagonal terms ``vals * y[col]``."""
+    rng = np.random.default_rng(seed)
+    n = core + tails * length
+    mask = np.eye(n, dtype=bool)
+    mask[:core, :core] = True
+    for t in range(tails):
+        base = core + t * length
+        for i in range(length):
+            if i:
+                mask[base + i, base + i - 1] = mask[base + i - 1, base + i] = True
+        mask[base, t % core] = mask[t % core, base] = True
+    rows, cols = np.nonzero(mask & ~np.eye(n, dtype=bool))
+    vals = rng.uniform(-1.0, 1.0, size=rows.size) / np.sqrt(core)
+    return rows, cols, vals, mask
+
+
+def bordered_system(core=45, tails=7, length=15, quad=0.01, seed=5):
+    """``ode_fn`` of dy/dt = A(p) y + quad * y*y over the bordered pattern."""
+    rows, cols, vals, mask = bordered_structure(core, tails, length, seed)
+    n = len(mask)
+    terms = [[f"p[{j}] * y[{j}]"] for j in range(n)]
+    for r, c, v in zip(rows, cols, vals):
+        terms[r].append(f"{float(v)!r} * y[{c}]")
+    components = [" + ".join(t) + f" + {float(quad)!r} * (y[{j}] * y[{j}])"
+                  for j, t in enumerate(terms)]
+    return make_tuple_callback("bordered", components), mask, (rows, cols, vals)
+
+
+def bordered_ensemble(size, n, seed=13):
+    """Diagonal entries -LogUniform[1, 1e4] (a stiff spread), y0 ~ U[-1, 1]."""
+    rng = np.random.default_rng(seed)
+    lam = -(10.0 ** rng.uniform(0.0, 4.0, size=(size, n)))
+    y0 = rng.uniform(-1.0, 1.0, size=(size, n))
+    return y0, lam

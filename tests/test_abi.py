@@ -14,7 +14,7 @@ import re
 import pytest
 
 from menodax_amd import _kernels, _runtime
-from menodax_amd import systems as S
+from tests import systems as S
 
 HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "menodax_hip.h")
 

@@ -15,7 +15,7 @@ import pytest
 
 from menodax_amd import _codegen as cg
 from menodax_amd import _kernels, _runtime
-from menodax_amd import systems as S
+from tests import systems as S
 from oracle import oracle as O
 
 CASES = [

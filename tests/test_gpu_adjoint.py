@@ -11,7 +11,8 @@ machinery, gradients independent of the checkpoint interval.
 import numpy as np
 import pytest
 
-from menodax_amd import rodas5P, systems as S
+from menodax_amd import rodas5P
+from tests import systems as S
 
 pytestmark = pytest.mark.gpu
 

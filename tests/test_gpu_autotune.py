@@ -3,7 +3,8 @@
 import numpy as np
 import pytest
 
-from menodax_amd import autotune, rodas5P, systems as S
+from menodax_amd import autotune, rodas5P
+from tests import systems as S
 
 pytestmark = pytest.mark.gpu
 

@@ -7,7 +7,8 @@ results and gradients equal the eager ones."""
 import numpy as np
 import pytest
 
-from menodax_amd import rodas5P, systems as S, tsit5
+from menodax_amd import rodas5P, tsit5
+from tests import systems as S
 
 pytestmark = pytest.mark.gpu
 

@@ -7,7 +7,8 @@ buffers give the eager result for those inputs."""
 import numpy as np
 import pytest
 
-from menodax_amd import rodas5P, systems as S
+from menodax_amd import rodas5P
+from tests import systems as S
 from menodax_amd.graph import CapturedSolve
 
 pytestmark = pytest.mark.gpu

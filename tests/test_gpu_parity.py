@@ -10,8 +10,19 @@ and both launch shapes. Published reference goldens are checked on the GPU too.
 import numpy as np
 import pytest
 
-from menodax_amd import rodas5P, systems as S, tsit5
+from menodax_amd import rodas5P, tsit5
+from tests import systems as S
 from oracle import oracle as O
+
+
+def _structure(mask, n, ordering):
+    """The oracle's own sparse structure for ``mask`` under the product's
+    elimination order (the AMD permutation is the one input it takes)."""
+    from menodax_amd import _sparse_direct as SD
+    from menodax_amd._sparsity import normalize_sparsity
+
+    pattern = normalize_sparsity(mask, n)
+    return O.sparse_structure(pattern, SD.fill_reducing_order(pattern, ordering))
 
 pytestmark = pytest.mark.gpu
 
@@ -190,7 +201,7 @@ def test_ring_with_fill_in_sparse_bit_exact(lu, ordering):
     kw = dict(rtol=1e-9, atol=1e-11, first_step=1e-6, lu_precision=lu)
     hist, st = rodas5P.solve(S.ring6, y0, t, p, sparsity=mask, ordering=ordering,
                              return_stats=True, **kw)
-    layout = SD.oracle_tables(normalize_sparsity(mask, 6), ordering)
+    layout = _structure(mask, 6, ordering)
     ref, rst = O.rodas5p(O.ring6(), y0, t, p, layout=layout, **kw)
     _same(hist, st, ref, rst)
     dense = rodas5P.solve(S.ring6, y0, t, p, **kw)
@@ -221,7 +232,7 @@ def test_vdp_chain_sparse_fp32_bit_exact(refill):
     t = np.linspace(0.0, 200.0, 11)
     hist, st = rodas5P.solve(fn, y0, t, mu, sparsity=mask, rtol=1e-6, atol=1e-8,
                              lu_precision="fp32", return_stats=True, refill=refill)
-    layout = SD.oracle_tables(normalize_sparsity(mask, 64), "amd")
+    layout = _structure(mask, 64, "amd")
     ref, rst = O.rodas5p(O.vdp_lattice(32, mu=1.0, d=0.1, wrap=False), y0, t, mu, layout=layout,
                          rtol=1e-6, atol=1e-8, lu_precision="fp32")
     _same(hist, st, ref, rst)
@@ -239,7 +250,7 @@ def test_block_arrow_sparse_bit_exact():
     for lu in ("fp32", "fp64"):
         hist, st = rodas5P.solve(fn, y0, t, lam, sparsity=mask, rtol=1e-6, atol=1e-8,
                                  lu_precision=lu, return_stats=True, refill=True)
-        layout = SD.oracle_tables(normalize_sparsity(mask, 50), "amd")
+        layout = _structure(mask, 50, "amd")
         ref, rst = O.rodas5p(system, y0, t, lam, layout=layout, rtol=1e-6, atol=1e-8,
                              lu_precision=lu)
         _same(hist, st, ref, rst)
@@ -259,7 +270,7 @@ def test_every_placement_is_bit_exact(placement, lu):
     y0, lam = S.arrow_ensemble(200, n=24)
     t = np.linspace(0.0, 10.0, 7)
     system = O.linquad(24, rows, cols, vals, 0.01, off_pidx=pidx, n_p=n_p)
-    layout = SD.oracle_tables(normalize_sparsity(mask, 24), "amd")
+    layout = _structure(mask, 24, "amd")
     ref, rst = O.rodas5p(system, y0, t, lam, layout=layout, rtol=1e-6, atol=1e-8, lu_precision=lu)
     for refill in (False, True):
         hist, st = rodas5P.solve(fn, y0, t, lam, sparsity=mask, rtol=1e-6, atol=1e-8,
@@ -396,7 +407,7 @@ def test_a_claim_order_changes_no_bit():
     y0, lam = S.arrow_ensemble(3000, n=24)
     t = np.linspace(0.0, 10.0, 7)
     system = O.linquad(24, rows, cols, vals, 0.01, off_pidx=pidx, n_p=n_p)
-    layout = SD.oracle_tables(normalize_sparsity(mask, 24), "amd")
+    layout = _structure(mask, 24, "amd")
     ref, rst = O.rodas5p(system, y0, t, lam, layout=layout, rtol=1e-6, atol=1e-8, lu_precision="fp32")
     kw = dict(sparsity=mask, rtol=1e-6, atol=1e-8, lu_precision="fp32", return_stats=True)
     cost = _common.cost_order(rst["accepted_steps"] + rst["rejected_steps"])
@@ -411,6 +422,42 @@ def test_a_claim_order_changes_no_bit():
         hist, st = tsit5.solve(S.lorenz, y0l, tl, pl, rtol=1e-6, atol=1e-8, return_stats=True,
                                work_order=order)
         _same(hist, st, ref, rst)
+
+
+@pytest.mark.parametrize("ordered", [False, True])
+@pytest.mark.parametrize("grid", [1, 7, 300])
+def test_refill_claims_every_trajectory_exactly_once(grid, ordered):
+    """The persistent launch's invariant. A lane claims until one claim lands
+    at or past N and then stops, and the counter's atomic adds hand out
+    disjoint ranges, so a final counter of N + (lanes in the grid) means the
+    claims covered [0, N) once each and every lane stopped after exactly one
+    failed claim. With that, the results equal the one-lane-per-trajectory
+    kernel's (every trajectory solved, none twice)."""
+    import torch
+
+    from menodax_amd import _common, _kernels
+
+    n = 3000
+    y0, p = S.robertson_lognormal_ensemble(n, seed=4)
+    t = S.robertson_benchmark_times()
+    built = _kernels.rodas5p_kernel(S.robertson, 3, 3, _kernels.KernelOptions())
+    dev = torch.device("cuda")
+    times = torch.tensor(t, device=dev)
+    y0_t, p_t = torch.tensor(y0, device=dev), torch.tensor(p, device=dev)
+    kw = dict(dt0=0.0, rtol=1e-6, atol=1e-8, max_steps=100000)
+    plain = rodas5P.launch(built, y0_t, times, p_t, n, refill=False, **kw)
+    ws = torch.full((256,), 0xAB, dtype=torch.uint8, device=dev)  # the launch must clear it
+    order = (torch.tensor(np.random.default_rng(grid).permutation(n), device=dev)
+             if ordered else None)
+    out = rodas5P.launch(built, y0_t, times, p_t, n, refill=True, grid=grid, workspace=ws,
+                         order=order, **kw)
+    torch.cuda.synchronize()
+    counter = int(ws[:8].view(torch.int64).item())
+    assert counter == n + grid * built.block
+    for a, b in ((out.hist, plain.hist), (out.accepted, plain.accepted),
+                 (out.rejected, plain.rejected), (out.loop, plain.loop)):
+        assert torch.equal(a, b)
+    assert _common.solver_stats(out.accepted, out.rejected, out.loop)["accepted_steps"].min() > 0
 
 
 def test_array_rhs_stage_evaluations_are_bit_identical():

@@ -10,7 +10,8 @@ final state. Every placement and both launch shapes agree bit for bit.
 import numpy as np
 import pytest
 
-from menodax_amd import rodas5P, systems as S
+from menodax_amd import rodas5P
+from tests import systems as S
 
 pytestmark = pytest.mark.gpu
 

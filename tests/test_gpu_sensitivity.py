@@ -5,10 +5,21 @@ and forward mode) that contract S, against closed forms and differences."""
 import numpy as np
 import pytest
 
-from menodax_amd import rodas5P, systems as S, tsit5
+from menodax_amd import rodas5P, tsit5
+from tests import systems as S
 from menodax_amd._sensitivity import SensitivitySpec
 from oracle import oracle as O
 from oracle.callbacks import HostCallbacks
+
+
+def _structure(mask, n, ordering):
+    """The oracle's own sparse structure for ``mask`` under the product's
+    elimination order (the AMD permutation is the one input it takes)."""
+    from menodax_amd import _sparse_direct as SD
+    from menodax_amd._sparsity import normalize_sparsity
+
+    pattern = normalize_sparsity(mask, n)
+    return O.sparse_structure(pattern, SD.fill_reducing_order(pattern, ordering))
 
 pytestmark = pytest.mark.gpu
 
@@ -67,7 +78,7 @@ def test_joint_sparse_parameter_columns_bit_exact_with_the_oracle():
                                                   sens_param_columns=cols, sparsity=mask,
                                                   return_stats=True, **kw)
     spec = SensitivitySpec(12, n_p, False, True, param_columns=cols)
-    layout = SD.oracle_tables(normalize_sparsity(mask, 12), "amd")
+    layout = _structure(mask, 12, "amd")
     ref_state, ref_sens, rst = _oracle_joint(HostCallbacks(fn, 12, n_p), y0, t, lam, spec,
                                              layout=layout, **kw)
     for k in rst:

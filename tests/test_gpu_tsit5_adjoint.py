@@ -11,7 +11,8 @@ torch backward of ``solve(..., adjoint=True)`` equal to ``adjoint_gradients``.
 import numpy as np
 import pytest
 
-from menodax_amd import systems as S, tsit5
+from menodax_amd import tsit5
+from tests import systems as S
 
 pytestmark = pytest.mark.gpu
 

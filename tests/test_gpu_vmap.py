@@ -4,7 +4,8 @@
 import numpy as np
 import pytest
 
-from menodax_amd import rodas5P, systems as S, tsit5
+from menodax_amd import rodas5P, tsit5
+from tests import systems as S
 
 pytestmark = pytest.mark.gpu
 

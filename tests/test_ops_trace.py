@@ -6,7 +6,8 @@ graph instead. The traced graphs are recorded, not run (no GPU here)."""
 import numpy as np
 import pytest
 
-from menodax_amd import rodas5P, systems as S, tsit5
+from menodax_amd import rodas5P, tsit5
+from tests import systems as S
 
 
 def _trace(f, *args):

@@ -20,8 +20,18 @@ step.
 import numpy as np
 import pytest
 
-from menodax_amd import systems as S
+from tests import systems as S
 from oracle import oracle as O
+
+
+def _structure(mask, n, ordering):
+    """The oracle's own sparse structure for ``mask`` under the product's
+    elimination order (the AMD permutation is the one input it takes)."""
+    from menodax_amd import _sparse_direct as SD
+    from menodax_amd._sparsity import normalize_sparsity
+
+    pattern = normalize_sparsity(mask, n)
+    return O.sparse_structure(pattern, SD.fill_reducing_order(pattern, ordering))
 
 LORENZ_GOLDEN = {  # divergence: (mean attempted, min, max, mean rejected)
     0.0: (722.0, 722, 722, 104.0),
@@ -143,7 +153,7 @@ def test_chain_closed_form_dense_and_sparse():
     t = np.array([0.0, 0.5, 2.0])
     kw = dict(rtol=1e-11, atol=1e-13, first_step=1e-4, lu_precision="fp64")
     dense, _ = O.rodas5p(O.chain4(), y0, t, p, **kw)
-    layout = SD.oracle_tables(normalize_sparsity(S.CHAIN4_MASK, 4), "natural")
+    layout = _structure(S.CHAIN4_MASK, 4, "natural")
     sparse, _ = O.rodas5p(O.chain4(), y0, t, p, layout=layout, **kw)
     np.testing.assert_allclose(sparse, dense, rtol=1e-10, atol=1e-12)
     np.testing.assert_allclose(sparse[0, -1], S.chain4_exact(2.0), rtol=1e-9, atol=1e-11)

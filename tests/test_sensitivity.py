@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from menodax_amd import _codegen as cg
-from menodax_amd import systems as S
+from tests import systems as S
 from menodax_amd._sensitivity import SensitivitySpec, augmented_y0, split_augmented
 from oracle import oracle as O
 from oracle.callbacks import HostCallbacks

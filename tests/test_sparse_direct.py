@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from menodax_amd import _sparse_direct as SD
-from menodax_amd import systems as S
+from tests import systems as S
 from menodax_amd._sparsity import (
     CompressedJacobian,
     check_orthogonal,
@@ -377,3 +377,172 @@ def test_colouring_matches_networkx_greedy_best_of_five():
                  "saturation_largest_first", "DSATUR")),
                key=lambda c: max(c.values()))
     assert got.n_colours == max(best.values()) + 1
+
+
+# --- the emitted device code compiled for the host ----------------------------
+_HOST_PRELUDE = r"""
+#include <cmath>
+#define __device__
+#define __constant__ static const
+#define __forceinline__ inline
+#define __restrict__ __restrict
+namespace mdx {
+inline double fmaT(double a, double b, double c) { return std::fma(a, b, c); }
+inline float fmaT(float a, float b, float c) { return std::fmaf(a, b, c); }
+}
+namespace sys {
+"""
+_HOST_EPILOGUE = r"""
+}  // namespace sys
+#define ENTRY(T, S)                                                              \
+  extern "C" void factor_##S(T* lu) { sys::sparse_factor(lu); }                 \
+  extern "C" void solve_##S(const T* lu, T* r) { sys::sparse_solve(lu, r); }    \
+  extern "C" void solve_t_##S(const T* lu, T* r) { sys::sparse_solve_t(lu, r); }
+ENTRY(double, d)
+ENTRY(float, f)
+"""
+
+
+def _host_build(source, tmp_path, name):
+    """The device source as a host shared library (g++, no contraction, so
+    every multiply-add is the explicit fma the device code names)."""
+    import ctypes
+    import subprocess
+
+    src = tmp_path / f"{name}.cpp"
+    lib = tmp_path / f"lib{name}.so"
+    src.write_text(_HOST_PRELUDE + source + _HOST_EPILOGUE)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-fPIC", "-shared",
+                    "-o", str(lib), str(src)], check=True)
+    return ctypes.CDLL(str(lib))
+
+
+def _host_run(lib, dtype, lu, b):
+    import ctypes
+
+    sfx = "d" if dtype == np.float64 else "f"
+    ct = ctypes.c_double if dtype == np.float64 else ctypes.c_float
+    lu = np.array(lu, dtype=dtype, copy=True)
+    getattr(lib, f"factor_{sfx}")(lu.ctypes.data_as(ctypes.POINTER(ct)))
+    x = np.array(b, dtype=dtype, copy=True)
+    xt = x.copy()
+    getattr(lib, f"solve_{sfx}")(lu.ctypes.data_as(ctypes.POINTER(ct)),
+                                 x.ctypes.data_as(ctypes.POINTER(ct)))
+    getattr(lib, f"solve_t_{sfx}")(lu.ctypes.data_as(ctypes.POINTER(ct)),
+                                   xt.ctypes.data_as(ctypes.POINTER(ct)))
+    return lu, x, xt
+
+
+@pytest.mark.parametrize("ordering", ["natural", "amd"])
+@pytest.mark.parametrize("mask", [_ring(9), S.arrow_structure(24)[4], S.vdp_sparsity(8, wrap=False),
+                                  _bordered_block(4, 2, 5)],
+                         ids=["ring9", "block-arrow24", "vdp-chain16", "bordered"])
+def test_table_driven_code_computes_what_the_unrolled_code_does(mask, ordering, tmp_path,
+                                                                monkeypatch):
+    """Both emissions of one analysis, compiled for the host: the factors and
+    both solves agree bit for bit in both precisions (the reference requires
+    the same of its two forms, tests/test_sparse_direct.py:385-415), and they
+    solve the matrix (numpy's solve and its transpose)."""
+    n = len(mask)
+    pattern = normalize_sparsity(mask, n)
+    solver = SD.sparse_direct_solver_for(pattern, ordering)
+    assert solver.unrolled() == (True, True)
+    unrolled = _host_build(solver.device_source(), tmp_path, "unrolled")
+    monkeypatch.setattr(SD, "MAX_UNROLLED_SUBSTITUTIONS", 0)
+    monkeypatch.setattr(SD, "MAX_UNROLLED_UPDATES", 0)
+    assert solver.unrolled() == (False, False)
+    table_src = solver.device_source()
+    assert "FACTOR_TAB" in table_src and "SOLVE_T_TAB" in table_src
+    tables = _host_build(table_src, tmp_path, "tables")
+    dense = _matrix(mask, seed=8)
+    lu0 = np.zeros(solver.nnz)
+    table = solver.layout.slot_table()
+    rows, cols = np.nonzero(table >= 0)
+    lu0[table[rows, cols]] = dense[rows, cols]
+    b = np.random.default_rng(9).normal(size=n)
+    for dtype in (np.float64, np.float32):
+        got_u = _host_run(unrolled, dtype, lu0, b)
+        got_t = _host_run(tables, dtype, lu0, b)
+        for u, t in zip(got_u, got_t):
+            assert u.tobytes() == t.tobytes()
+    _, x, xt = got_u
+    _, x64, xt64 = _host_run(tables, np.float64, lu0, b)
+    np.testing.assert_allclose(x64, np.linalg.solve(dense, b), rtol=1e-10)
+    np.testing.assert_allclose(xt64, np.linalg.solve(dense.T, b), rtol=1e-10)
+    np.testing.assert_allclose(x, np.linalg.solve(dense, b), rtol=1e-4)
+
+
+def test_a_large_bordered_block_is_table_driven_by_itself(tmp_path):
+    """150 variables with a dense 45-variable core: 29,475 updates and 2,190
+    substitutions, over both thresholds without lowering them; the emitted
+    tables solve the matrix."""
+    fn, mask, _ = S.bordered_system()
+    n = len(mask)
+    solver = SD.sparse_direct_solver_for(normalize_sparsity(mask, n), "amd")
+    m, u, s = solver.layout.flop_counts()
+    assert u > SD.MAX_UNROLLED_UPDATES and s > SD.MAX_UNROLLED_SUBSTITUTIONS
+    assert solver.unrolled() == (False, False)
+    lib = _host_build(solver.device_source(), tmp_path, "bordered")
+    dense = _matrix(mask, seed=2, shift=30.0)
+    lu0 = np.zeros(solver.nnz)
+    table = solver.layout.slot_table()
+    rows, cols = np.nonzero(table >= 0)
+    lu0[table[rows, cols]] = dense[rows, cols]
+    b = np.random.default_rng(3).normal(size=n)
+    _, x, xt = _host_run(lib, np.float64, lu0, b)
+    np.testing.assert_allclose(x, np.linalg.solve(dense, b), rtol=1e-9)
+    np.testing.assert_allclose(xt, np.linalg.solve(dense.T, b), rtol=1e-9)
+
+
+# --- the product's analysis against the oracle's restatement -----------------
+def _random_masks():
+    rng = np.random.default_rng(21)
+    out = []
+    for n in (5, 12, 30, 60):
+        for density in (1.5, 3.0):
+            mask = rng.random((n, n)) < density / n  # unsymmetric on purpose
+            out.append(mask)
+    return out
+
+
+@pytest.mark.parametrize("ordering", ["natural", "amd"])
+@pytest.mark.parametrize("mask", MASKS + [S.arrow_structure(50)[4], S.vdp_sparsity(32, wrap=False),
+                                          S.bordered_structure()[3]] + _random_masks())
+def test_the_analysis_equals_the_oracles_elimination_on_the_boolean_pattern(mask, ordering):
+    """The product's up-looking symbolic factorisation against the oracle's
+    right-looking elimination on the boolean pattern (two independent
+    constructions of one mathematically unique structure): CSR, diagonal
+    positions and the caller slot table all equal, for the same order."""
+    from oracle import oracle as O
+
+    n = len(mask)
+    pattern = normalize_sparsity(mask, n)
+    layout = SD.analyse(pattern, ordering)
+    nnz, row_ptr, col_ind, diag_ptr, order, slot_of, _ = O.sparse_structure(pattern, layout.order)
+    assert nnz == layout.nnz
+    np.testing.assert_array_equal(row_ptr, layout.row_ptr)
+    np.testing.assert_array_equal(col_ind, layout.col_ind)
+    np.testing.assert_array_equal(diag_ptr, layout.diag_ptr)
+    np.testing.assert_array_equal(order, layout.row_origin)
+    np.testing.assert_array_equal(slot_of, layout.slot_table().ravel())
+
+
+def test_the_oracles_fill_is_what_numeric_elimination_touches():
+    """The oracle's boolean elimination against unpivoted elimination on
+    random numbers over unsymmetric patterns (no cancellation at these
+    densities): every structural entry is touched and nothing else."""
+    from oracle import oracle as O
+
+    for mask in _random_masks():
+        n = len(mask)
+        order = tuple(np.random.default_rng(n).permutation(n))
+        nnz, row_ptr, col_ind, *_ = O.sparse_structure(normalize_sparsity(mask, n), order)
+        work = _matrix(mask, seed=n)[np.ix_(order, order)]
+        for k in range(n):
+            work[k + 1:, k] /= work[k, k]
+            work[k + 1:, k + 1:] -= np.outer(work[k + 1:, k], work[k, k + 1:])
+        touched = work != 0.0
+        expected = np.zeros((n, n), dtype=bool)
+        for i in range(n):
+            expected[i, col_ind[row_ptr[i]:row_ptr[i + 1]]] = True
+        assert (touched == expected).all()
