@@ -171,6 +171,33 @@ int32_t mdx_adjoint_launch(int64_t handle, void* stream, int64_t n_traj, int32_t
                            int32_t ckpt_every, int32_t max_ckpt, void* workspace,
                            int64_t workspace_bytes);
 
+/* Bytes of device scratch behind mdx_adjoint_launch_static (per device). */
+#define MDX_STATIC_SCRATCH_BYTES (64ll << 20)
+
+/* mdx_adjoint_launch without a workspace: the checkpoints and step records go
+ * to a static device array of MDX_STATIC_SCRATCH_BYTES that the library holds
+ * on each device it runs on until the process ends. No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops that forbid allocation). A call whose
+ * mdx_adjoint_workspace_bytes(n_traj, n_vars, ckpt_every, max_ckpt) exceeds
+ * MDX_STATIC_SCRATCH_BYTES is rejected before anything runs. It calls no
+ * allocator: its first call on a device resolves the array's address there,
+ * and (as any launch) loads the kernel's code object if no earlier call on
+ * that device has, so a caller that must not allocate at all makes that first
+ * call (or mdx_kernel_info) before it starts. Every call on a device shares
+ * that device's one array, which keeps the last call's contents (the kernel
+ * writes each checkpoint before it reads it, so nothing needs clearing):
+ * calls that overlap on one device race, so issue them on one stream or wait
+ * for one before the next. Arguments and results otherwise as
+ * mdx_adjoint_launch, bit for bit. */
+int32_t mdx_adjoint_launch_static(int64_t handle, void* stream, int64_t n_traj, int32_t n_vars,
+                                  int32_t n_params, int32_t n_save, int32_t layout,
+                                  const double* y0, const double* times, const double* params,
+                                  double dt0, double rtol, double atol, int32_t max_steps,
+                                  const double* weights, const double* gbar, double* grad_y0,
+                                  double* grad_p, int32_t* status, int32_t ckpt_every,
+                                  int32_t max_ckpt);
+
 /* Device facts used for code generation and the roofline: architecture name
  * (e.g. "gfx950") and compute-unit count of the current device. */
 int32_t mdx_device_info(char* arch, size_t arch_cap, int32_t* n_cu);

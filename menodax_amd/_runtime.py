@@ -106,6 +106,8 @@ def lib():
             vp, vp, vp, vp, i32, i32, vp, i64,
         ]
         L.mdx_adjoint_launch.restype = i32
+        L.mdx_adjoint_launch_static.argtypes = L.mdx_adjoint_launch.argtypes[:-2]
+        L.mdx_adjoint_launch_static.restype = i32
         L.mdx_device_info.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(i32)]
         L.mdx_device_info.restype = i32
         if L.mdx_abi_version() != 1:
@@ -285,6 +287,28 @@ def adjoint_launch(handle, stream, n_traj, n_vars, n_params, n_save, layout, y0,
             int(max_ckpt), workspace, int(workspace_bytes),
         ),
         "mdx_adjoint_launch",
+    )
+
+
+#: bytes of the static device scratch behind ``adjoint_launch_static``
+#: (``MDX_STATIC_SCRATCH_BYTES`` in include/menodax_hip.h)
+STATIC_SCRATCH_BYTES = 64 << 20
+
+
+def adjoint_launch_static(handle, stream, n_traj, n_vars, n_params, n_save, layout, y0, times,
+                          params, dt0, rtol, atol, max_steps, weights, gbar, grad_y0, grad_p,
+                          status, ckpt_every, max_ckpt):
+    """Launch through ``mdx_adjoint_launch_static``: as ``adjoint_launch``, with
+    the checkpoints in the library's static device scratch instead of a
+    workspace (nothing is allocated)."""
+    _check(
+        lib().mdx_adjoint_launch_static(
+            handle, stream, int(n_traj), int(n_vars), int(n_params), int(n_save), int(layout),
+            y0, times, params or None, float(dt0), float(rtol), float(atol), int(max_steps),
+            weights or None, gbar, grad_y0, grad_p or None, status, int(ckpt_every),
+            int(max_ckpt),
+        ),
+        "mdx_adjoint_launch_static",
     )
 
 

@@ -26,6 +26,14 @@
 
 #include "menodax_hip.h"
 
+// Device scratch of mdx_adjoint_launch_static, the entry point that allocates
+// nothing: the adjoint's checkpoints and step records live here instead of in a
+// caller's workspace. Each device that runs the library holds its own copy
+// until the process ends. In host code `scratch` names a host-side stand-in,
+// never the device array: kernels get the address hipGetSymbolAddress resolves
+// for the current device (static_scratch below), and only that address.
+static __device__ __align__(256) unsigned char scratch[MDX_STATIC_SCRATCH_BYTES];
+
 namespace {
 
 // Must match mdx::SolveArgs in menodax_amd/kernels/mdx_common.hpp member for
@@ -103,6 +111,8 @@ struct Kernel {
 std::mutex g_mutex;
 std::vector<std::unique_ptr<Kernel>> g_kernels;  // handle = index + 1; never freed
 
+std::map<int, void*> g_scratch;  // device -> resolved address of `scratch`
+
 int32_t lookup(int64_t handle, Kernel** out) {
   std::lock_guard<std::mutex> lock(g_mutex);
   if (handle < 1 || handle > (int64_t)g_kernels.size())
@@ -135,6 +145,35 @@ int32_t loaded_on_current_device(Kernel* k, Loaded** out) {
   *out = &res.first->second;
   return 0;
 }
+
+// The address of `scratch` on the current device, resolved on the first call
+// there (with that device current) and kept.
+int32_t static_scratch(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lock(g_mutex);
+  auto it = g_scratch.find(dev);
+  if (it != g_scratch.end()) {
+    *out = it->second;
+    return 0;
+  }
+  void* ptr = nullptr;
+  e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  if (!ptr) return fail(1, "hipGetSymbolAddress(scratch) returned a null address");
+  g_scratch.emplace(dev, ptr);
+  *out = ptr;
+  return 0;
+}
+
+int32_t adjoint_impl(int64_t handle, void* stream, int64_t n_traj, int32_t n_vars,
+                     int32_t n_params, int32_t n_save, int32_t layout, const double* y0,
+                     const double* times, const double* params, double dt0, double rtol,
+                     double atol, int32_t max_steps, const double* weights, const double* gbar,
+                     double* grad_y0, double* grad_p, int32_t* status, int32_t ckpt_every,
+                     int32_t max_ckpt, void* workspace, int64_t workspace_bytes,
+                     bool use_static, const char* who);
 
 }  // namespace
 
@@ -437,25 +476,65 @@ int32_t mdx_adjoint_launch(int64_t handle, void* stream, int64_t n_traj, int32_t
                            const double* gbar, double* grad_y0, double* grad_p, int32_t* status,
                            int32_t ckpt_every, int32_t max_ckpt, void* workspace,
                            int64_t workspace_bytes) {
+  return adjoint_impl(handle, stream, n_traj, n_vars, n_params, n_save, layout, y0, times, params,
+                      dt0, rtol, atol, max_steps, weights, gbar, grad_y0, grad_p, status,
+                      ckpt_every, max_ckpt, workspace, workspace_bytes, false,
+                      "mdx_adjoint_launch");
+}
+
+int32_t mdx_adjoint_launch_static(int64_t handle, void* stream, int64_t n_traj, int32_t n_vars,
+                                  int32_t n_params, int32_t n_save, int32_t layout,
+                                  const double* y0, const double* times, const double* params,
+                                  double dt0, double rtol, double atol, int32_t max_steps,
+                                  const double* weights, const double* gbar, double* grad_y0,
+                                  double* grad_p, int32_t* status, int32_t ckpt_every,
+                                  int32_t max_ckpt) {
+  return adjoint_impl(handle, stream, n_traj, n_vars, n_params, n_save, layout, y0, times, params,
+                      dt0, rtol, atol, max_steps, weights, gbar, grad_y0, grad_p, status,
+                      ckpt_every, max_ckpt, nullptr, 0, true, "mdx_adjoint_launch_static");
+}
+
+}  // extern "C"
+
+namespace {
+int32_t adjoint_impl(int64_t handle, void* stream, int64_t n_traj, int32_t n_vars,
+                     int32_t n_params, int32_t n_save, int32_t layout, const double* y0,
+                     const double* times, const double* params, double dt0, double rtol,
+                     double atol, int32_t max_steps, const double* weights, const double* gbar,
+                     double* grad_y0, double* grad_p, int32_t* status, int32_t ckpt_every,
+                     int32_t max_ckpt, void* workspace, int64_t workspace_bytes,
+                     bool use_static, const char* who) {
+  const std::string w(who);
   Kernel* k = nullptr;
   if (int32_t rc = lookup(handle, &k)) return rc;
-  if (!k->adjoint) return fail(1, "mdx_adjoint_launch: not an adjoint kernel");
+  if (!k->adjoint) return fail(1, w + ": not an adjoint kernel");
   if (n_traj < 0 || n_vars < 1 || n_params < 0 || n_save < 1)
-    return fail(1, "mdx_adjoint_launch: bad extents");
-  if (layout != 0 && layout != 1) return fail(1, "mdx_adjoint_launch: layout must be 0 or 1");
+    return fail(1, w + ": bad extents");
+  if (layout != 0 && layout != 1) return fail(1, w + ": layout must be 0 or 1");
   if (ckpt_every < 1 || max_ckpt < 1)
-    return fail(1, "mdx_adjoint_launch: ckpt_every and max_ckpt must be positive");
+    return fail(1, w + ": ckpt_every and max_ckpt must be positive");
   if (!y0 || !times || (!params && n_params > 0) || !gbar || !grad_y0 ||
-      (!grad_p && n_params > 0) || !status || !workspace)
-    return fail(1, "mdx_adjoint_launch: null buffer");
+      (!grad_p && n_params > 0) || !status || (!use_static && !workspace))
+    return fail(1, w + ": null buffer");
   if (!weights && !k->unit_weights)
-    return fail(1, "mdx_adjoint_launch: this kernel reads error weights; weights is NULL");
-  if (max_steps < 0) return fail(1, "mdx_adjoint_launch: max_steps must be non-negative");
+    return fail(1, w + ": this kernel reads error weights; weights is NULL");
+  if (max_steps < 0) return fail(1, w + ": max_steps must be non-negative");
   const int64_t need = mdx_adjoint_workspace_bytes(n_traj, n_vars, ckpt_every, max_ckpt);
-  if (workspace_bytes < need)
-    return fail(1, "mdx_adjoint_launch: workspace holds " + std::to_string(workspace_bytes) +
+  if (use_static) {
+    if (need > (int64_t)MDX_STATIC_SCRATCH_BYTES)
+      return fail(1, w + ": the checkpoints need " + std::to_string(need) +
+                         " bytes of scratch, more than the static scratch's " +
+                         std::to_string((long long)MDX_STATIC_SCRATCH_BYTES) +
+                         " (fewer trajectories per call, a larger ckpt_every, or "
+                         "mdx_adjoint_launch with a workspace)");
+  } else if (workspace_bytes < need) {
+    return fail(1, w + ": workspace holds " + std::to_string(workspace_bytes) +
                        " bytes, needs " + std::to_string(need));
+  }
   if (n_traj == 0) return 0;
+  if (use_static) {
+    if (int32_t rc = static_scratch(&workspace)) return rc;
+  }
 
   AdjointArgs g;
   std::memset(&g, 0, sizeof(g));
@@ -495,13 +574,16 @@ int32_t mdx_adjoint_launch(int64_t handle, void* stream, int64_t n_traj, int32_t
   Loaded* l = nullptr;
   if (int32_t rc = loaded_on_current_device(k, &l)) return rc;
   const int64_t blocks = (n_traj + k->block - 1) / k->block;
-  if (blocks > 0x7fffffff) return fail(1, "mdx_adjoint_launch: grid too large");
+  if (blocks > 0x7fffffff) return fail(1, w + ": grid too large");
   void* kargs[] = {&g};
   hipError_t e = hipModuleLaunchKernel(l->fn, (unsigned)blocks, 1, 1, (unsigned)k->block, 1, 1, 0,
                                        (hipStream_t)stream, kargs, nullptr);
   if (e != hipSuccess) return hip_fail(e, ("hipModuleLaunchKernel(" + k->entry + ")").c_str());
   return 0;
 }
+}  // namespace
+
+extern "C" {
 
 int32_t mdx_device_info(char* arch, size_t arch_cap, int32_t* n_cu) {
   int dev = 0;

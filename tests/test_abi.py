@@ -137,3 +137,26 @@ def test_last_error_is_thread_local_text():
     lib = _runtime.lib()
     assert isinstance(lib.mdx_last_error(), bytes)
     assert ctypes.c_int32(lib.mdx_abi_version()).value == 1
+
+
+def test_the_static_adjoint_rejects_a_request_beyond_its_scratch():
+    """mdx_adjoint_launch_static refuses, before touching the GPU, checkpoints
+    that would not fit its static device scratch."""
+    fn = S.vdp_lattice(1, mu=1.0, d=0.1, wrap=False)[0]
+    adj = _kernels.rodas5p_adjoint_kernel(fn, 2, 1, _kernels.KernelOptions(lu_precision="fp64"))
+    # one checkpoint (2 + 8 words) and one step record (2 + 4 words) per trajectory
+    fits = _runtime.STATIC_SCRATCH_BYTES // (8 * (10 + 6))
+    assert _runtime.adjoint_workspace_bytes(fits, 2, 1, 1) <= _runtime.STATIC_SCRATCH_BYTES
+    assert _runtime.adjoint_workspace_bytes(fits + 1, 2, 1, 1) > _runtime.STATIC_SCRATCH_BYTES
+    with pytest.raises(_runtime.NativeError, match="static scratch"):
+        _runtime.adjoint_launch_static(adj.plain, None, fits + 1, 2, 1, 20, 1, 8, 8, 8, 0.0,
+                                       1e-8, 1e-10, 100, 0, 8, 8, 8, 8, 1, 1)
+    with pytest.raises(_runtime.NativeError, match="null buffer"):
+        _runtime.adjoint_launch_static(adj.plain, None, 4, 2, 1, 20, 1, 8, 8, 8, 0.0,
+                                       1e-8, 1e-10, 100, 0, 8, 0, 8, 8, 1, 1)
+
+
+def test_the_static_scratch_size_matches_the_header():
+    text = open(HEADER).read()
+    m = re.search(r"#define MDX_STATIC_SCRATCH_BYTES \((\d+)ll << (\d+)\)", text)
+    assert m and int(m.group(1)) << int(m.group(2)) == _runtime.STATIC_SCRATCH_BYTES
