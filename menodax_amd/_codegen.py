@@ -1226,6 +1226,26 @@ def generate_sensitivity(traced: Traced):
     return "\n\n".join(parts) + "\n"
 
 
+def generate_jvp(traced: Traced):
+    """``jvp(y, t, p, v, out)``: ``out[i] = sum_j df_i/dy_j v_j``, the Jacobian
+    times a vector by one forward sweep seeded with ``v`` (read through an
+    accessor), each ``out[i]`` assigned a double (a pointer or an accessor
+    whose elements take the assignment). The refined FP32 stage solve uses it
+    for its residual; returns (source, flops)."""
+    g = traced.graph
+    n = traced.n_vars
+    nodes = tangents(g, traced.outputs, {g.y(j).id: g.inp("v", j) for j in range(n)})
+    lines, fc = Emitter(g).body(nodes)
+    outs = [f"  out[{j}] = {value_of(x)};" for j, x in enumerate(nodes)]
+    src = "\n".join(
+        ["template <class PA, class YA, class VA, class OA>",
+         "__device__ __forceinline__ void jvp(const YA& y, const double t, const PA& p,",
+         "                                    const VA& v, OA out) {",
+         "  (void)t;", "  (void)p;", "  (void)v;"]
+        + lines + outs + ["}"])
+    return src + "\n", fc
+
+
 @dataclass
 class AdjointNodes:
     """The derivative contractions a discrete adjoint of Rodas5P needs, as

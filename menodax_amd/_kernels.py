@@ -92,6 +92,10 @@ class KernelOptions:
     hook_size: int = 0
     save_history: bool = True
     hook_n_save: int = 0
+    #: with an FP32 LU: refine each stage solve once against the FP64 iteration
+    #: matrix and keep the stage vectors in FP64 (no counterpart in the
+    #: reference, whose FP32 path stores the stage vectors in FP32)
+    lu_refine: bool = False
 
 
 #: the unrolled Rodas5P step keeps its tableau in vector registers up to this
@@ -132,7 +136,7 @@ def tableau_in_registers(n_vars: int, n_traj: int, n_cu: int, placement=None,
 
 
 def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None,
-                      dense_matrix: int = 0) -> dict:
+                      dense_matrix: int = 0, k_bytes: int | None = None) -> dict:
     """The placement macros for a Rodas5P kernel.
 
     Up to ``UNROLLED_MAX_VARS`` the whole step stays in registers. Above it,
@@ -142,7 +146,9 @@ def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None
     element about n/3 times a step), the eight stage vectors (about 50 element
     reads per component per step), the stage argument and the state; whatever
     does not fit in one compute unit's LDS next to them stays in scratch.
-    Outputs are bit-identical across placements.
+    Outputs are bit-identical across placements. ``k_bytes`` is a stage
+    vector element's size when it differs from the LU's (refined FP32 solves
+    keep them in FP64).
     """
     if placement is None:
         placement = "registers" if n_vars <= UNROLLED_MAX_VARS else "lds"
@@ -166,7 +172,7 @@ def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None
     if placement == "lds":
         lu_bytes = 4 if lu_precision == "fp32" else 8
         for macro, per_lane in (("MDX_LDS_LU", dense_matrix * lu_bytes),
-                                ("MDX_LDS_K", 8 * n_vars * lu_bytes),
+                                ("MDX_LDS_K", 8 * n_vars * (k_bytes or lu_bytes)),
                                 ("MDX_LDS_U", 8 * n_vars), ("MDX_LDS_Y", 8 * n_vars)):
             if per_lane and lds + per_lane * block <= LDS_BYTES_PER_CU:
                 defs[macro] = 1
@@ -177,7 +183,7 @@ def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None
 
 
 def rodas5p_block(requested, n_aug: int, lu_precision: str, n_traj: int, n_cu: int,
-                  placement=None, sparsity=None) -> int:
+                  placement=None, sparsity=None, lu_refine=False) -> int:
     """The workgroup size of a plain or joint Rodas5P solve: the caller's, else
     a wavefront, except for a small ensemble of a sparse system whose rolled
     step (``n_aug``, the joint vector's length, above ``UNROLLED_MAX_VARS``;
@@ -195,7 +201,7 @@ def rodas5p_block(requested, n_aug: int, lu_precision: str, n_traj: int, n_cu: i
         return trajectories_per_block_or_default(requested)
     if placement not in (None, "lds") or sparsity is None or n_aug <= UNROLLED_MAX_VARS:
         return DEFAULT_BLOCK
-    lu_bytes = 4 if lu_precision == "fp32" else 8
+    lu_bytes = 4 if lu_precision == "fp32" and not lu_refine else 8
     stage_vectors = 8 * n_aug * lu_bytes
     if stage_vectors * DEFAULT_BLOCK <= LDS_BYTES_PER_CU:
         return DEFAULT_BLOCK
@@ -331,6 +337,14 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
     never LDS)."""
     if options.lu_precision not in ("fp32", "fp64"):
         raise ValueError(f"lu_precision must be 'fp32' or 'fp64', got {options.lu_precision!r}")
+    if options.lu_refine:
+        if options.lu_precision != "fp32":
+            raise ValueError("lu_refine refines FP32 solves; it needs lu_precision='fp32'")
+        if adjoint or options.sens is not None:
+            raise ValueError("lu_refine is for a plain solve: the adjoint and the forward "
+                             "sensitivities differentiate the unrefined step")
+        if options.placement == WAVE:
+            raise ValueError("lu_refine runs one trajectory per lane, not the wave placement")
     traced = _codegen.trace(ode_fn, n_vars, n_params)
     layout = None
     if options.sparsity is None:
@@ -388,7 +402,8 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
     # when it fits (the forward step only)
     dense_matrix = (structure.size if (options.sparsity is None and n_vars > DENSE_UNROLL_MAX
                                        and not adjoint) else 0)
-    place = rodas5p_placement(n_aug, options.lu_precision, block, placement, dense_matrix)
+    place = rodas5p_placement(n_aug, options.lu_precision, block, placement, dense_matrix,
+                              k_bytes=8 if options.lu_refine else None)
     defs = {
         "MDX_N": n_vars,
         "MDX_NP": n_params,
@@ -415,6 +430,7 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
         "MDX_MAX_VGPRS": _max_vgprs(options.max_registers),
         "MDX_TAB_REGS": 1 if (place["MDX_ROLLED"] == 0 and options.tableau_in_registers
                               and not adjoint and spec is None) else 0,
+        "MDX_REFINE": 1 if options.lu_refine else 0,
     }
     sens_src = ""
     if options.save_hook is not None or not options.save_history:
@@ -437,14 +453,17 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
         defs["MDX_SENS_PCOLS"] = ", ".join(str(c) for c in spec.direction_columns)
         defs["MDX_SENS_UNROLL"] = spec.n_sens if spec.n_sens <= 4 else 1
         sens_src = _codegen.generate_sensitivity(traced)
+    jvp_src, jvp_fc = "", None
+    if options.lu_refine:
+        jvp_src, jvp_fc = _codegen.generate_jvp(traced)
     lines = _prelude(defs)
-    lines += ["namespace sys {", gen.source, sens_src, sparse_src, "}  // namespace sys",
+    lines += ["namespace sys {", gen.source, sens_src, jvp_src, sparse_src, "}  // namespace sys",
               '#include "rodas5p.hpp"']
     if adjoint:
         lines.append('#include "rodas5p_adjoint.hpp"')
     source = "\n".join(lines) + "\n"
     return source, dict(gen=gen, structure=structure, layout=layout, lu_f=lu_f, solve_f=solve_f,
-                        block=block, placement=place, n_aug=n_aug)
+                        block=block, placement=place, n_aug=n_aug, jvp_flops=jvp_fc)
 
 
 @functools.lru_cache(maxsize=None)
@@ -461,7 +480,7 @@ def rodas5p_kernel(ode_fn, n_vars: int, n_params: int, options: KernelOptions) -
         rhs_flops=d["gen"].rhs_flops, jac_flops=d["gen"].jac_flops, structure=d["structure"],
         lu_flops=d["lu_f"], solve_flops=d["solve_f"], lu_precision=options.lu_precision,
         layout=d["layout"], extra={"placement": d["placement"], "n_aug": d["n_aug"],
-                                   "sens": options.sens},
+                                   "sens": options.sens, "jvp_flops": d["jvp_flops"]},
     )
 
 

@@ -48,14 +48,17 @@ class Choice:
     tableau_in_registers: bool = False
 
 
-def candidates(n_vars: int, lu_precision: str, blocks=(64, 128, 256), dense=False):
+def candidates(n_vars: int, lu_precision: str, blocks=(64, 128, 256), dense=False,
+               lu_refine=False):
     """(placement, block, tableau in registers) worth timing: the unrolled
     register kernel only where it can plausibly fit, with and without the
     tableau in registers where that fits too, the LDS placement only at block
     sizes where something fits in LDS, and for a dense matrix above the
     unrolled size the wave placement (one trajectory per 64-lane workgroup)."""
     out = []
-    if dense and _kernels.DENSE_UNROLL_MAX < n_vars <= _kernels.WAVE_MAX_VARS:
+    k_bytes = 8 if lu_refine else None
+    if (dense and not lu_refine
+            and _kernels.DENSE_UNROLL_MAX < n_vars <= _kernels.WAVE_MAX_VARS):
         out.append((_kernels.WAVE, 64, False))
     for placement in PLACEMENTS:
         if placement == "registers" and n_vars > 32:
@@ -63,7 +66,8 @@ def candidates(n_vars: int, lu_precision: str, blocks=(64, 128, 256), dense=Fals
         tabs = (False, True) if (placement == "registers"
                                  and n_vars <= _kernels.TABLEAU_REGS_MAX_VARS) else (False,)
         for b in blocks:
-            if placement == "lds" and rodas5p_placement(n_vars, lu_precision, b, "lds")["lds_bytes"] == 0:
+            if placement == "lds" and rodas5p_placement(n_vars, lu_precision, b, "lds",
+                                                        k_bytes=k_bytes)["lds_bytes"] == 0:
                 continue
             out.extend((placement, b, tab) for tab in tabs)
     return out
@@ -71,7 +75,8 @@ def candidates(n_vars: int, lu_precision: str, blocks=(64, 128, 256), dense=Fals
 
 def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-10,
          first_step=None, max_steps=100000, lu_precision="fp32", sparsity=None, ordering="amd",
-         pcoeff=0.0, icoeff=1.0, dcoeff=0.0, tf_index=None, blocks=(64, 128, 256), shapes=None):
+         pcoeff=0.0, icoeff=1.0, dcoeff=0.0, tf_index=None, blocks=(64, 128, 256), shapes=None,
+         lu_refine=False):
     """Time every candidate on ``sample`` trajectories (all of them by default;
     see the module docstring for the launch shape of a sample) and return
     ``(best Choice, [Choice for every candidate])``. Raises if two candidates
@@ -89,7 +94,8 @@ def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-
         sparsity = normalize_sparsity(sparsity, n_vars)
     base = KernelOptions(pcoeff=float(pcoeff), icoeff=float(icoeff), dcoeff=float(dcoeff),
                          lu_precision=lu_precision, sparsity=sparsity, ordering=ordering,
-                         tf_index=-1 if tf_index is None else int(tf_index))
+                         tf_index=-1 if tf_index is None else int(tf_index),
+                         lu_refine=bool(lu_refine))
     if shapes not in (None, "solve"):
         raise ValueError(f"shapes must be None or 'solve', got {shapes!r}")
     key = (ode_fn, n_vars, n_params, base, n, n_full, tuple(blocks), shapes)
@@ -99,7 +105,7 @@ def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-
     kw = dict(dt0=_common.initial_step(first_step), rtol=rtol, atol=atol, max_steps=max_steps)
     results, reference = [], None
     for placement, block, tab in candidates(n_vars, lu_precision, blocks,
-                                            dense=sparsity is None):
+                                            dense=sparsity is None, lu_refine=lu_refine):
         built = _kernels.rodas5p_kernel(
             ode_fn, n_vars, n_params,
             replace(base, placement=placement, trajectories_per_block=block,

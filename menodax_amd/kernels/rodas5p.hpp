@@ -30,6 +30,10 @@
 //   MDX_HOOK, MDX_HOOK_SIZE  call sys::save_hook at every save with the saved
 //                        state, accumulating into the trajectory's hook_out row
 //   MDX_SAVE_HISTORY     0: write no history, only the final state in row 0
+//   MDX_REFINE           1 (with MDX_LU_FP32): each stage solve takes one step
+//                        of iterative refinement against the double iteration
+//                        matrix (sys::jvp forms J x) and the stage vectors are
+//                        stored in double
 //
 // The method, stage by stage, is the reference kernel's (modax/rodas5P.py:
 // tableau 44-105, controller 108-113, 340-342, loop 761-989, tail 991-1005);
@@ -46,6 +50,16 @@ namespace r5 {
 typedef float LUT;
 #else
 typedef double LUT;
+#endif
+#ifndef MDX_REFINE
+#define MDX_REFINE 0
+#endif
+// The stage vectors' storage: the LU type (the reference's typing), or double
+// when the FP32 solves are refined.
+#if MDX_REFINE
+typedef double KT;
+#else
+typedef LUT KT;
 #endif
 
 constexpr int N = MDX_N;
@@ -197,6 +211,9 @@ struct Shifted {
 #if MDX_COOP && (MDX_HOOK || MDX_ADJOINT || MDX_BLOCK != 64)
 #error "a wave-per-trajectory kernel is a forward solve (plain or joint) on one 64-lane wave"
 #endif
+#if MDX_REFINE && !(MDX_LU_FP32 && MDX_NSENS == 0 && !MDX_COOP && !MDX_ADJOINT)
+#error "refined stage solves are for a plain forward solve with an FP32 LU on one lane"
+#endif
 #if MDX_COOP
 #define MDX_LSTRIDE 1
 // row stride of the shared matrix: one padding element keeps a column's
@@ -218,7 +235,7 @@ __device__ __forceinline__ unsigned opaque_lane() {
 #define MDX_LANE() (void)0
 #endif
 #if MDX_LDS_K
-__shared__ LUT lds_k[8 * NA * MDX_LSTRIDE];
+__shared__ KT lds_k[8 * NA * MDX_LSTRIDE];
 #define KQ(q, j) lds_k[((q) * NA + (j)) * MDX_LSTRIDE + lane_]
 #else
 #define KQ(q, j) k[q][j]
@@ -884,6 +901,54 @@ __device__ __forceinline__ void sens_stage(int S, const Z& ybase, double t, cons
 #endif
 }
 
+#if MDX_REFINE
+// Stage S's stored vector as a Jacobian-vector product's direction.
+template <class KS>
+struct StageVec {
+  KS ks;
+  int S;
+  __device__ __forceinline__ double operator[](int j) const { return ks(S, j); }
+};
+
+// The refinement's right-hand side as J x is produced: element j of J x lands
+// in r[j] = (LUT)(fma(-x_j, 1/(h gamma), b_j) + (J x)_j), the residual
+// b - M x against M = I/(h gamma) - J in double, x_j = ks(S, j).
+template <class KS>
+struct Residual {
+  LUT* r;
+  const double* b;
+  KS ks;
+  int S;
+  double gi;
+  struct Elem {
+    LUT* r;
+    double b, x, gi;
+    __device__ __forceinline__ void operator=(double c) const {
+      *r = (LUT)(fmaT(-x, gi, b) + c);
+    }
+  };
+  __device__ __forceinline__ Elem operator[](int j) const {
+    return Elem{r + j, b[j], ks(S, j), gi};
+  }
+};
+
+// One step of mixed-precision iterative refinement of stage S's solve. On
+// entry ks(S, .) holds x, the FP32 solution of the rounded system; the
+// residual of b (the stage's right-hand side in double) against the double
+// iteration matrix at the step's base point is solved with the same FP32
+// factors and added, so the stored stage vector is x + dx in double.
+template <class YB, class KS, class L>
+__device__ __forceinline__ void refine(int S, const YB& ybase, double t, const PA& p, double gi,
+                                       const double* __restrict__ b, KS ks, const L& lu,
+                                       const Pivots& ipiv) {
+  LUT r[N];
+  sys::jvp(ybase, t, p, StageVec<KS>{ks, S}, Residual<KS>{r, b, ks, S, gi});
+  solve(lu, ipiv, r);
+#pragma unroll
+  for (int j = 0; j < N; ++j) ks(S, j) = ks(S, j) + (double)r[j];
+}
+#endif
+
 // The tableau as the unrolled step reads it: compile-time constants, or
 // (MDX_TAB_REGS) copies made once per kernel in registers the compiler cannot
 // fold back into constants. Only the entries a step reads are kept live.
@@ -925,11 +990,16 @@ struct Tab {
 // (sensitivity blocks) keep their value in double for sens_stage.
 template <int S>
 __device__ __forceinline__ void stage(const double* __restrict__ work,
-                                      const double* __restrict__ dT, LUT (*__restrict__ k)[NA],
+                                      const double* __restrict__ dT, KT (*__restrict__ k)[NA],
                                       const LUT* __restrict__ lu, const Pivots& ipiv,
                                       double dt_use, double inv_dt, double* __restrict__ sv,
-                                      const Tab& tb) {
+                                      const Tab& tb, const double* __restrict__ ybase, double t,
+                                      const PA& p, double gi) {
+  (void)ybase, (void)t, (void)p, (void)gi;
   LUT r[N];
+#if MDX_REFINE
+  double b[N];
+#endif
 #pragma unroll
   for (int j = 0; j < NA; ++j) {
     double v;
@@ -944,20 +1014,27 @@ __device__ __forceinline__ void stage(const double* __restrict__ work,
       else
         v = fmaT(acc, inv_dt, work[j]);
     }
-    if (j < N)
+    if (j < N) {
       r[j] = (LUT)v;
-    else
+#if MDX_REFINE
+      b[j] = v;
+#endif
+    } else {
       sv[j - N] = v;
+    }
   }
   solve(lu, ipiv, r);
 #pragma unroll
   for (int j = 0; j < N; ++j) k[S][j] = r[j];
+#if MDX_REFINE
+  refine(S, ybase, t, p, gi, b, [&](int q, int j) -> KT& { return k[q][j]; }, lu, ipiv);
+#endif
 }
 
 // u = y + sum_q A_{S+1,q} k_q (rodas5P.py:649-661).
 template <int S>
 __device__ __forceinline__ void advance(double* __restrict__ u, const double* __restrict__ y,
-                                        LUT (*__restrict__ k)[NA], const Tab& tb) {
+                                        KT (*__restrict__ k)[NA], const Tab& tb) {
 #pragma unroll
   for (int j = 0; j < NA; ++j) {
     double acc = tb.a(S, 0) * (double)k[0][j];
@@ -1030,7 +1107,7 @@ __device__ __forceinline__ void chunk_sum(const double* __restrict__ c, int S, i
 #pragma unroll 1
   for (int q = 1; q < S; ++q) {
     const double cq = c[q];
-    LUT kv[MDX_CHUNK];
+    KT kv[MDX_CHUNK];
 #pragma unroll
     for (int jj = 0; jj < MDX_CHUNK; ++jj)
       if (jj < count) kv[jj] = ks(q, j0 + jj);
@@ -1092,16 +1169,26 @@ __device__ __forceinline__ void rolled_stage(int S, const Z& z, double ts, const
                                              double t, const PA& p,
                                              const double* __restrict__ dT, KS ks, KST kst,
                                              const L& lu, const Pivots& ipiv, double dt_use,
-                                             double inv_dt) {
-  (void)ybase, (void)t;
+                                             double inv_dt, double gi) {
+  (void)ybase, (void)t, (void)gi;
   double work[N];
   sys::rhs(z, ts, p, work);
   LUT r[N];
+#if MDX_REFINE
+  // the right-hand side replaces f in work, element by element (each row's f
+  // is read before its right-hand side is written), for the residual
+  stage_rows(S, 0, work, dT, kst, dt_use, inv_dt,
+             [&](int i, double v) { r[i] = (LUT)v; work[i] = v; });
+#else
   stage_rows(S, 0, work, dT, kst, dt_use, inv_dt,
              [&](int i, double v) { r[i] = (LUT)v; });
+#endif
   solve(lu, ipiv, r);
 #pragma unroll
   for (int j = 0; j < N; ++j) ks(S, j) = r[j];
+#if MDX_REFINE
+  refine(S, ybase, t, p, gi, work, ks, lu, ipiv);
+#endif
 #if MDX_NSENS > 0
 #pragma unroll MDX_SENS_UNROLL
   for (int d = 0; d < NS; ++d) {
@@ -1158,10 +1245,10 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
   factor(lu, ipiv);
 
 #if !MDX_LDS_K
-  LUT k[8][NA];
-  auto ks = [&](int q, int j) -> LUT& { return k[q][j]; };
+  KT k[8][NA];
+  auto ks = [&](int q, int j) -> KT& { return k[q][j]; };
 #else
-  auto ks = [&](int q, int j) -> LUT& { return KQ(q, j); };
+  auto ks = [&](int q, int j) -> KT& { return KQ(q, j); };
 #endif
 #if !MDX_LDS_U
   double u[NA];
@@ -1182,7 +1269,7 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
   for (int S = 0; S < 8; ++S) {
     MDX_LANE();
     // the stage vectors through this stage's lane register
-    auto kst = [&](int q, int j) -> LUT { return KQ(q, j); };
+    auto kst = [&](int q, int j) -> KT { return KQ(q, j); };
 #if !MDX_LDS_K
     MDX_PIN(k[S & 7], NA);
 #endif
@@ -1230,7 +1317,7 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
 #pragma unroll
       for (int j0 = 0; j0 < NA; j0 += MDX_CHUNK) {
         double uv[MDX_CHUNK];
-        LUT kv[MDX_CHUNK];
+        KT kv[MDX_CHUNK];
 #pragma unroll
         for (int jj = 0; jj < MDX_CHUNK; ++jj)
           if (j0 + jj < NA) uv[jj] = UJ(j0 + jj), kv[jj] = KQ(S - 1, j0 + jj);
@@ -1243,49 +1330,49 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
 #if MDX_LDS_U && MDX_LDS_Y
     // the first stage evaluates at the state itself, read where it lies
     rolled_stage(S, LdsVec{S == 0 ? &lds_y[lane_] : &lds_u[lane_]}, ts, ybase, t, p, dT, ks,
-                 kst, lu, ipiv, dt_use, inv_dt);
+                 kst, lu, ipiv, dt_use, inv_dt, dtgamma_inv);
 #elif MDX_LDS_U
     rolled_stage(S, LdsVec{&lds_u[lane_]}, ts, ybase, t, p, dT, ks, kst, lu, ipiv, dt_use,
-                 inv_dt);
+                 inv_dt, dtgamma_inv);
 #else
-    rolled_stage(S, u, ts, ybase, t, p, dT, ks, kst, lu, ipiv, dt_use, inv_dt);
+    rolled_stage(S, u, ts, ybase, t, p, dT, ks, kst, lu, ipiv, dt_use, inv_dt, dtgamma_inv);
 #endif
   }
 #undef MDX_PIN
 #else
   double work[NA];
   rhs_aug(s.y, t, p, work);
-  stage<0>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
+  stage<0>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb, s.y, t, p, dtgamma_inv);
   sens_stage(0, ybase, t, p, sv, lu, ipiv, ks);
   advance<0>(u, s.y, k, tb);
   rhs_aug(u, t + tb.t(0) * dt_use, p, work);
-  stage<1>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
+  stage<1>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb, s.y, t, p, dtgamma_inv);
   sens_stage(1, ybase, t, p, sv, lu, ipiv, ks);
   advance<1>(u, s.y, k, tb);
   rhs_aug(u, t + tb.t(1) * dt_use, p, work);
-  stage<2>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
+  stage<2>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb, s.y, t, p, dtgamma_inv);
   sens_stage(2, ybase, t, p, sv, lu, ipiv, ks);
   advance<2>(u, s.y, k, tb);
   rhs_aug(u, t + tb.t(2) * dt_use, p, work);
-  stage<3>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
+  stage<3>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb, s.y, t, p, dtgamma_inv);
   sens_stage(3, ybase, t, p, sv, lu, ipiv, ks);
   advance<3>(u, s.y, k, tb);
   rhs_aug(u, t + tb.t(3) * dt_use, p, work);
-  stage<4>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
+  stage<4>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb, s.y, t, p, dtgamma_inv);
   sens_stage(4, ybase, t, p, sv, lu, ipiv, ks);
   advance<4>(u, s.y, k, tb);
   rhs_aug(u, t_end, p, work);
-  stage<5>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
+  stage<5>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb, s.y, t, p, dtgamma_inv);
   sens_stage(5, ybase, t, p, sv, lu, ipiv, ks);
 #pragma unroll
   for (int j = 0; j < NA; ++j) u[j] = u[j] + (double)k[5][j];
   rhs_aug(u, t_end, p, work);
-  stage<6>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
+  stage<6>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb, s.y, t, p, dtgamma_inv);
   sens_stage(6, ybase, t, p, sv, lu, ipiv, ks);
 #pragma unroll
   for (int j = 0; j < NA; ++j) u[j] = u[j] + (double)k[6][j];
   rhs_aug(u, t_end, p, work);
-  stage<7>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb);
+  stage<7>(work, dT, k, lu, ipiv, dt_use, inv_dt, sv, tb, s.y, t, p, dtgamma_inv);
   sens_stage(7, ybase, t, p, sv, lu, ipiv, ks);
 #endif
 
@@ -1369,7 +1456,7 @@ __device__ __forceinline__ void step(State& s, PA& p, const SolveArgs& a, i64 i,
 #pragma unroll
     for (int j0 = 0; j0 < NA; j0 += MDX_CHUNK) {
       double uv[MDX_CHUNK];
-      LUT kv[MDX_CHUNK];
+      KT kv[MDX_CHUNK];
 #pragma unroll
       for (int jj = 0; jj < MDX_CHUNK; ++jj)
         if (j0 + jj < NA) uv[jj] = UJ(j0 + jj), kv[jj] = KQ(7, j0 + jj);

@@ -89,6 +89,13 @@ def rodas5p_step(built) -> Work:
     else:
         fp64 += lu + solves + diag
         fp32 = 0
+    # refined stage solves (lu_refine): per stage one Jacobian-vector product
+    # in double, the residual (an fma and an add per component), a second FP32
+    # solve and the correction's add
+    jvp = built.extra.get("jvp_flops") if built.extra else None
+    if jvp is not None:
+        fp64 += 8 * (jvp.flops + 4 * n)
+        fp32 += 8 * built.solve_flops
     # the accepted-step state update (counted on every attempt: an upper bound
     # of n flops on rejected steps)
     fp64 += n

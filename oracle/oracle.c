@@ -586,7 +586,12 @@ typedef struct {
   int* ipiv;
 } owork;
 
+/* lu_fp32 == 2: FP32 factorisation and solves, each stage solve refined once
+ * against the double iteration matrix, stage vectors stored in double */
+static int refined(const orun* R) { return R->lu_fp32 == 2; }
+
 static double kget(const orun* R, const owork* W, int s, int j) {
+  if (refined(R)) return W->kd[s * R->na + j];
   return R->lu_fp32 ? (double)W->ks[s * R->na + j] : W->kd[s * R->na + j];
 }
 
@@ -669,7 +674,7 @@ static void stage(const orun* R, const olin* L, const osys* S, owork* W, int s,
         value = fma(acc, inv_dt, W->work[j]);
     }
     if (j >= n) W->work[j] = value;
-    else if (R->lu_fp32) W->rhss[j] = (float)value;
+    else if (R->lu_fp32) { W->rhss[j] = (float)value; W->rhsd[j] = value; }
     else W->rhsd[j] = value;
   }
   for (int blk = 0; blk <= R->n_sens; ++blk) {
@@ -684,7 +689,24 @@ static void stage(const orun* R, const olin* L, const osys* S, owork* W, int s,
         if (R->lu_fp32) W->rhss[j] = (float)v; else W->rhsd[j] = v;
       }
     }
-    if (R->lu_fp32) {
+    if (refined(R)) {
+      /* x solves the rounded system in FP32; the residual of the stage's
+       * double right-hand side b against the double matrix
+       * M = I/(h gamma) - J(y_n, t_n), r = fma(-x, 1/(h gamma), b) + J x
+       * (J the closed-form Jacobian the step evaluated, summed in column
+       * order), is solved with the same FP32 factors; k = x + dx in double */
+      const double gi = 1.0 / (dt_use * GAMMA);
+      double* x = W->kd + s * na + base;
+      if (L->sparse) sparse_solve_s(L, W->lus, W->rhss); else dense_solve_s(n, W->lus, W->ipiv, W->rhss);
+      for (int j = 0; j < n; ++j) x[j] = (double)W->rhss[j];
+      for (int j = 0; j < n; ++j) {
+        double jx = 0.0;
+        for (int c = 0; c < n; ++c) jx = jx + W->J[j * n + c] * x[c];
+        W->rhss[j] = (float)(fma(-x[j], gi, W->rhsd[j]) + jx);
+      }
+      if (L->sparse) sparse_solve_s(L, W->lus, W->rhss); else dense_solve_s(n, W->lus, W->ipiv, W->rhss);
+      for (int j = 0; j < n; ++j) x[j] = x[j] + (double)W->rhss[j];
+    } else if (R->lu_fp32) {
       if (L->sparse) sparse_solve_s(L, W->lus, W->rhss); else dense_solve_s(n, W->lus, W->ipiv, W->rhss);
       for (int j = 0; j < n; ++j) W->ks[s * na + base + j] = W->rhss[j];
     } else {
@@ -971,6 +993,7 @@ int oracle_rodas5p_cb(void* cb_f, void* cb_jac, void* cb_sens_f, void* cb_sens_d
   S.cb_sens_dt = (cb_sens_t)cb_sens_dt;
   S.cb_sens_cpl = (cb_cpl_t)cb_sens_cpl;
   if (n_sens > 0 && (!cb_sens_f || !cb_sens_dt || !cb_sens_cpl)) return -10;
+  if (n_sens > 0 && lu_fp32 == 2) return -11; /* refinement: plain solves only */
   orun R = {n, n_p, n_save, times, dt0, rtol, atol, max_steps, lu_fp32,
             R_EXPONENT * (icoeff + pcoeff + dcoeff), -R_EXPONENT * (pcoeff + 2.0 * dcoeff),
             R_EXPONENT * dcoeff, tf_index, n_sens, n * (1 + n_sens), n_error, pcol};

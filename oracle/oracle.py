@@ -242,9 +242,13 @@ def _ensemble(system, y0, params, weights):
 
 def rodas5p(system, y0, times, params, *, weights=None, first_step=None, rtol=1e-8,
             atol=1e-10, max_steps=100000, lu_precision="fp32", pcoeff=0.0, icoeff=1.0,
-            dcoeff=0.0, tf_index=None, layout=None, threads=0):
+            dcoeff=0.0, tf_index=None, layout=None, threads=0, lu_refine=False):
     """Ensemble Rodas5P on the CPU. ``layout`` is a sparse ``(nnz, row_ptr, col_ind,
-    diag_ptr, order, slot_of, pattern)`` tuple, or ``None`` for the dense LU."""
+    diag_ptr, order, slot_of, pattern)`` tuple, or ``None`` for the dense LU.
+    ``lu_refine`` (FP32 LU only): one step of iterative refinement of every stage
+    solve against the double iteration matrix, stage vectors kept in double."""
+    if lu_refine and lu_precision != "fp32":
+        raise ValueError("lu_refine refines FP32 solves")
     lib = _lib()
     n_traj, y0, params, weights = _ensemble(system, y0, params, weights)
     times = np.ascontiguousarray(times, dtype=np.float64)
@@ -271,8 +275,9 @@ def rodas5p(system, y0, times, params, *, weights=None, first_step=None, rtol=1e
         pi(ints), pd(dbls), pi(off_rc), pd(off_vals), pd(inv_eps), I(n_traj), pd(y0),
         pd(times), I(times.size), pd(params), pd(weights),
         D(0.0 if first_step is None else float(first_step)), D(rtol), D(atol), I(max_steps),
-        I(1 if lu_precision == "fp32" else 0), D(pcoeff), D(icoeff), D(dcoeff),
-        I(-1 if tf_index is None else int(tf_index)), I(int(nnz)), pi(row_ptr), pi(col_ind),
+        I((2 if lu_refine else 1) if lu_precision == "fp32" else 0), D(pcoeff), D(icoeff),
+        D(dcoeff), I(-1 if tf_index is None else int(tf_index)), I(int(nnz)), pi(row_ptr),
+        pi(col_ind),
         pi(diag_ptr), pi(order), pi(slot_of), pi(pattern), pd(hist), pi(acc), pi(rej),
         pi(loop), I(int(threads)),
     )

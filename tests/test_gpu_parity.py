@@ -220,6 +220,29 @@ def test_chain_sparse_matches_closed_form():
     np.testing.assert_allclose(sparse[0, -1], S.chain4_exact(2.0), rtol=1e-9, atol=1e-11)
 
 
+def test_vdp_chain_full_window_mu_stratified_bit_exact():
+    """The configured C3 window, t in [0, 2000], on 8 trajectories spread over
+    mu in [10, 1000] (the 16,384-trajectory ensemble sorted by mu): the small-mu
+    ones run into max_steps = 100,000 (their window holds ~125 relaxation
+    periods of 32 oscillators out of phase), the large-mu ones finish; both
+    are bit-exact with the oracle, tail fill included."""
+    fn, _ = S.vdp_lattice(32, mu=1.0, d=0.1, wrap=False)
+    mask = S.vdp_sparsity(32, wrap=False)
+    y0, mu = S.vdp_chain_ensemble(32, 16384)
+    idx = np.argsort(mu[:, 0], kind="stable")[np.linspace(0, 16383, 8).astype(int)]
+    t = np.linspace(0.0, 2000.0, 11)
+    print(f"vdp chain full window: mu {np.round(mu[idx, 0], 1)}", flush=True)
+    hist, st = rodas5P.solve(fn, y0[idx], t, mu[idx], sparsity=mask, rtol=1e-6, atol=1e-8,
+                             lu_precision="fp32", return_stats=True, max_steps=100000)
+    layout = _structure(mask, 64, "amd")
+    ref, rst = O.rodas5p(O.vdp_lattice(32, mu=1.0, d=0.1, wrap=False), y0[idx], t, mu[idx],
+                         layout=layout, rtol=1e-6, atol=1e-8, lu_precision="fp32",
+                         max_steps=100000)
+    _same(hist, st, ref, rst)
+    assert np.asarray(st["loop_steps"]).max() == 100000  # the cap is reached
+    assert np.asarray(st["loop_steps"]).min() < 100000  # and not everywhere
+
+
 @pytest.mark.parametrize("refill", [False, True])
 def test_vdp_chain_sparse_fp32_bit_exact(refill):
     """The stiff VdP chain (64 variables, block-tridiagonal), sparse FP32 LU with AMD."""

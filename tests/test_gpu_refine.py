@@ -1,0 +1,113 @@
+"""Refined FP32 stage solves (``lu_refine=True``) on an MI355X.
+
+The mode has no reference counterpart; what pins it:
+  - the oracle's restatement of the refinement (``oracle.rodas5p(...,
+    lu_refine=True)``), itself pinned on the CPU by converging to the FP64-LU
+    solution (tests/test_oracle.py). The kernel forms J x with the generated
+    forward sweep and the oracle with its closed-form Jacobian summed in column
+    order, so the residuals may differ in their last double bits; the residual
+    is rounded to float before its solve, which absorbs nearly all of that.
+    The bar: every step counter equal and histories within 1e-10 (absolute,
+    on states of order one), with the share of bit-identical trajectories
+    printed;
+  - bit-identity across the kernel's own placements and launch shapes (the
+    same arithmetic in the same order);
+  - the property the mode exists for: the history follows the FP64-LU
+    kernel's to 1e-6, where the unrefined FP32 solve does not on the uneven
+    stiff ensemble.
+"""
+
+import numpy as np
+import pytest
+
+from menodax_amd import rodas5P
+from tests import systems as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _close(tag, hist, st, ref, rst):
+    for k in rst:
+        np.testing.assert_array_equal(np.asarray(st[k]), rst[k], err_msg=f"{tag} {k}")
+    same = np.all(hist == ref, axis=tuple(range(1, hist.ndim)))
+    print(f"{tag}: {same.mean() * 100:.1f}% of trajectories bit-identical to the oracle, "
+          f"max abs {np.abs(hist - ref).max():.2e}", flush=True)
+    np.testing.assert_allclose(hist, ref, rtol=0, atol=TOL)
+
+
+@pytest.mark.parametrize("placement", [None, "rolled"])
+def test_robertson_refined_against_the_oracle(placement):
+    y0, p = S.robertson_lognormal_ensemble(2000, seed=1)
+    t = S.robertson_benchmark_times()
+    hist, st = rodas5P.solve(S.robertson, y0, t, p, rtol=1e-6, atol=1e-8, lu_precision="fp32",
+                             lu_refine=True, placement=placement, return_stats=True)
+    ref, rst = O.rodas5p(O.robertson(), y0, t, p, rtol=1e-6, atol=1e-8, lu_precision="fp32",
+                         lu_refine=True)
+    _close(f"robertson refined placement={placement}", hist, st, ref, rst)
+
+
+def test_robertson_refined_placements_and_refill_bit_identical():
+    y0, p = S.robertson_lognormal_ensemble(4096, seed=2)
+    t = S.robertson_benchmark_times()
+    kw = dict(rtol=1e-6, atol=1e-8, lu_precision="fp32", lu_refine=True, return_stats=True)
+    base, bst = rodas5P.solve(S.robertson, y0, t, p, **kw)
+    for extra in (dict(placement="rolled"), dict(refill=True), dict(tableau_in_registers=False),
+                  dict(tableau_in_registers=True)):
+        h, s = rodas5P.solve(S.robertson, y0, t, p, **kw, **extra)
+        np.testing.assert_array_equal(h, base, err_msg=str(extra))
+        for k in bst:
+            np.testing.assert_array_equal(s[k], bst[k], err_msg=f"{extra} {k}")
+
+
+def test_robertson_refined_follows_the_fp64_kernel():
+    y0, p = S.robertson_lognormal_ensemble(8192, seed=3)
+    t = S.robertson_benchmark_times()
+    kw = dict(rtol=1e-6, atol=1e-8, return_stats=True)
+    h64, s64 = rodas5P.solve(S.robertson, y0, t, p, lu_precision="fp64", **kw)
+    hr, sr = rodas5P.solve(S.robertson, y0, t, p, lu_precision="fp32", lu_refine=True, **kw)
+    assert np.abs(hr - h64).max() < 1e-6
+    assert np.mean(sr["accepted_steps"] == s64["accepted_steps"]) > 0.99
+
+
+def _uneven(count):
+    from menodax_amd import _sparse_direct as SD
+    from menodax_amd._sparsity import normalize_sparsity
+
+    fn, mask, (rows, cols, vals, pidx) = S.arrow_system(50)
+    sp = normalize_sparsity(mask, 50)
+    system = O.linquad(50, rows, cols, vals, 0.01, off_pidx=pidx, n_p=S.arrow_n_params(50))
+    layout = O.sparse_structure(sp, SD.fill_reducing_order(sp, "amd"))
+    y0, p = S.arrow_ensemble(100000, n=50)
+    idx = np.linspace(0, 99999, count).astype(int)
+    return fn, mask, system, layout, y0[idx], p[idx], np.linspace(0.0, 10.0, 20)
+
+
+@pytest.mark.parametrize("placement,block", [("lds", None), ("lds", 32), ("rolled", None)])
+def test_uneven_stiff_refined_against_the_oracle(placement, block):
+    """The 50-variable block arrow (sparse LU, AMD) over its whole window on
+    256 trajectories spread across the 100,000-trajectory ensemble."""
+    fn, mask, system, layout, y0, p, t = _uneven(256)
+    hist, st = rodas5P.solve(fn, y0, t, p, rtol=1e-6, atol=1e-8, lu_precision="fp32",
+                             lu_refine=True, sparsity=mask, placement=placement,
+                             trajectories_per_block=block, return_stats=True)
+    ref, rst = O.rodas5p(system, y0, t, p, layout=layout, rtol=1e-6, atol=1e-8,
+                         lu_precision="fp32", lu_refine=True)
+    _close(f"uneven stiff refined {placement}/{block}", hist, st, ref, rst)
+    h64, _ = O.rodas5p(system, y0, t, p, layout=layout, rtol=1e-6, atol=1e-8,
+                       lu_precision="fp64")
+    assert np.abs(hist[:, -1] - h64[:, -1]).max() < 1e-6
+
+
+def test_refine_is_rejected_where_it_does_not_apply():
+    y0, p = S.robertson_lognormal_ensemble(4, seed=1)
+    t = S.robertson_benchmark_times()
+    with pytest.raises(ValueError, match="fp32"):
+        rodas5P.solve(S.robertson, y0, t, p, lu_precision="fp64", lu_refine=True)
+    torch = pytest.importorskip("torch")
+    y0_t = torch.tensor(y0, device="cuda", requires_grad=True)
+    with pytest.raises(ValueError, match="plain solve"):
+        rodas5P.solve(S.robertson, y0_t, torch.tensor(t), torch.tensor(p, device="cuda"),
+                      lu_refine=True)

@@ -256,3 +256,53 @@ def test_dense_lu_against_numpy():
     for r, c, v, k in zip(rows, cols, vals, pidx):
         A[r, c] += v * p[k] if k >= 0 else v
     np.testing.assert_allclose(out[0, -1], expm(0.5 * A) @ y0[0], rtol=1e-8, atol=1e-10)
+
+
+def _uneven_sample(count):
+    """``count`` trajectories spread over the 100,000-trajectory uneven stiff
+    ensemble (50-variable block arrow), with its oracle system and AMD layout."""
+    from menodax_amd import _sparse_direct as SD
+    from menodax_amd._sparsity import normalize_sparsity
+
+    _, mask, (rows, cols, vals, pidx) = S.arrow_system(50)
+    sp = normalize_sparsity(mask, 50)
+    system = O.linquad(50, rows, cols, vals, 0.01, off_pidx=pidx, n_p=S.arrow_n_params(50))
+    layout = O.sparse_structure(sp, SD.fill_reducing_order(sp, "amd"))
+    y0, p = S.arrow_ensemble(100000, n=50)
+    idx = np.linspace(0, 99999, count).astype(int)
+    return system, layout, y0[idx], p[idx], np.linspace(0.0, 10.0, 20)
+
+
+def test_refined_fp32_solves_follow_the_fp64_solution():
+    """Iterative refinement converges to the solution of the double system: one
+    refinement of every FP32 stage solve brings the solve to the FP64-LU
+    solve's trajectory (the same step counts, endpoints within 1e-7), where the
+    unrefined FP32 solve, whose stage vectors carry FP32 rounding, lands up to
+    1e-5 away on the long oscillating trajectories. A residual with a wrong
+    sign or without its J x term would not converge and stays at the
+    unrefined distance or beyond."""
+    system, layout, y0, p, times = _uneven_sample(64)
+    kw = dict(layout=layout, rtol=1e-6, atol=1e-8)
+    h64, s64 = O.rodas5p(system, y0, times, p, lu_precision="fp64", **kw)
+    h32, _ = O.rodas5p(system, y0, times, p, lu_precision="fp32", **kw)
+    hr, sr = O.rodas5p(system, y0, times, p, lu_precision="fp32", lu_refine=True, **kw)
+    d_ref = np.abs(hr[:, -1] - h64[:, -1]).max()
+    d_plain = np.abs(h32[:, -1] - h64[:, -1]).max()
+    assert d_plain > 1e-6
+    assert d_ref < 1e-7 and d_ref < d_plain / 50
+    assert np.array_equal(sr["accepted_steps"], s64["accepted_steps"])
+    assert np.array_equal(sr["rejected_steps"], s64["rejected_steps"])
+
+
+def test_refined_fp32_robertson_dense():
+    """The same on the dense path: Robertson (C2 inputs), every step count equal
+    to the FP64-LU solve's and the history within 5e-7 of it."""
+    y0, p = S.robertson_lognormal_ensemble(256, seed=1)
+    times = S.robertson_benchmark_times()
+    kw = dict(rtol=1e-6, atol=1e-8)
+    h64, s64 = O.rodas5p(O.robertson(), y0, times, p, lu_precision="fp64", **kw)
+    hr, sr = O.rodas5p(O.robertson(), y0, times, p, lu_precision="fp32", lu_refine=True, **kw)
+    assert np.array_equal(sr["accepted_steps"], s64["accepted_steps"])
+    assert np.abs(hr - h64).max() < 5e-7
+    with pytest.raises(ValueError, match="FP32"):
+        O.rodas5p(O.robertson(), y0, times, p, lu_precision="fp64", lu_refine=True)
