@@ -315,7 +315,11 @@ class Sym:
 
 
 class Cond:
-    """A traced comparison; usable only as the first argument of ``where``."""
+    """A traced comparison: the first argument of ``where``, or, in arithmetic,
+    the number 1.0 or 0.0 that Python makes of a bool (``a + (b - a) * (x > c)``,
+    the branchless select the reference's callbacks write); it is recorded as
+    ``select(cond, 1.0, 0.0)`` and then multiplied or added exactly as the
+    number would be."""
 
     __slots__ = ("g", "n")
 
@@ -328,6 +332,36 @@ class Cond:
             "the right-hand side branches on a traced comparison; use "
             "menodax_amd.where(cond, a, b), min, max or abs instead"
         )
+
+    def _num(self):
+        return Sym(self.g, self.g.select(self.n, self.g.const(1.0), self.g.const(0.0)))
+
+    def __add__(self, o):
+        return self._num() + (o._num() if isinstance(o, Cond) else o)
+
+    def __radd__(self, o):
+        return o + self._num()
+
+    def __sub__(self, o):
+        return self._num() - (o._num() if isinstance(o, Cond) else o)
+
+    def __rsub__(self, o):
+        return o - self._num()
+
+    def __mul__(self, o):
+        return self._num() * (o._num() if isinstance(o, Cond) else o)
+
+    def __rmul__(self, o):
+        return o * self._num()
+
+    def __truediv__(self, o):
+        return self._num() / (o._num() if isinstance(o, Cond) else o)
+
+    def __rtruediv__(self, o):
+        return o / self._num()
+
+    def __neg__(self):
+        return -self._num()
 
 
 def _int_power(g: Graph, a: Node, e: int) -> Node:

@@ -45,6 +45,7 @@
  *   factorisation forms inv_piv and the multiplier in f64 and stores f32; dense
  *   and sparse solves pure f32; sparse pivot reciprocal formed in f64, stored f32.
  */
+#include <complex.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -133,7 +134,55 @@ enum {
   SYS_DAMPED_OSC = 8,   /* tests/test_save_hook.py:23-24                       */
   SYS_KAPS = 9,         /* reference/systems/python/kaps.py:26-48             */
   SYS_CALLBACK = 10,    /* any system: f, dense J and df/dt from host callbacks */
+  SYS_BBN = 11,         /* examples/bbn_estimation/main.py:74-161, x = Q/T the time */
 };
+
+/* The four-species BBN network (n, p, D, 4He) with x = Q/T as the independent
+ * variable and parameters (log10 eta_10, N_eff): constants and rates as in
+ * examples/bbn_estimation/main.py:44-60, 74-161. Written once in complex
+ * arithmetic: with real arguments it is f itself; with a small imaginary
+ * perturbation of one input it gives that input's derivative column by the
+ * complex step, Im f(z + ih) / h, with no cancellation (an independent
+ * derivative method from the product's forward-mode sweeps). The g_star step
+ * at T = m_e is taken on the real part. */
+static void bbn_eval(const double complex* y, double complex x, const double* p,
+                     double complex* f) {
+  const double Q = 1.293, B_D = 2.225, M_N = 938.272, M_E = 0.511, M_PL = 1.2209e22;
+  const double ZETA3 = 1.2020569, N_EFF_SM = 3.044, TAU = 879.4 / 6.582119e-22;
+  const double S_NP = 4.0e-11, S_DD = 3.4e-5, PI = 3.14159265358979323846;
+  const double eta = pow(10.0, p[0] - 10.0);
+  const double complex T = Q / x;
+  const double g_star = 3.91 + (creal(T) > M_E ? 10.75 - 3.91 : 0.0) + 1.75 * (p[1] - N_EFF_SM);
+  const double complex H = sqrt(4.0 * PI * PI * PI * g_star / 45.0) * (T * T) / M_PL;
+  const double complex n_b = eta * (2.0 * ZETA3 / (PI * PI)) * (T * T * T);
+  const double complex x2 = x * x;
+  const double complex g_np = (255.0 / TAU) * (12.0 + 6.0 * x + x2) / (x2 * x2 * x) + 1.0 / TAU;
+  const double complex g_pn = g_np * cexp(-x);
+  const double complex v = 4.0 * PI / (M_N * T);
+  const double complex k_d = n_b * 0.75 * (v * csqrt(v)) * cexp(B_D / T);
+  const double complex r_np = n_b * S_NP * (y[0] * y[1] - y[2] / k_d);
+  const double complex r_dd = n_b * S_DD * (y[2] * y[2]);
+  const double complex den = H * x;
+  f[0] = (g_pn * y[1] - g_np * y[0] - r_np) / den;
+  f[1] = (g_np * y[0] - g_pn * y[1] - r_np) / den;
+  f[2] = (r_np - 2.0 * r_dd) / den;
+  f[3] = r_dd / den;
+}
+
+/* complex-step derivatives of the BBN network: J[r*4+c] and df/dx */
+static void bbn_jac(const double* y, double x, const double* p, double* J, double* dT) {
+  const double h = 1e-30;
+  double complex yc[4], f[4];
+  for (int c = 0; c <= 4; ++c) {
+    for (int j = 0; j < 4; ++j) yc[j] = y[j];
+    double complex xc = x;
+    if (c < 4) yc[c] += h * I; else xc += h * I;
+    bbn_eval(yc, xc, p, f);
+    for (int r = 0; r < 4; ++r) {
+      if (c < 4) J[r * 4 + c] = cimag(f[r]) / h; else dT[r] = cimag(f[r]) / h;
+    }
+  }
+}
 
 /* Host callbacks of a SYS_CALLBACK system, and the forward-sensitivity
  * derivatives of any system a joint solve carries: with a state direction s and
@@ -261,6 +310,13 @@ static void sys_f(const osys* s, const double* y, double t, const double* p, dou
       f[0] = y[1];
       f[1] = -p[0] * y[0] - p[1] * y[1];
       break;
+    case SYS_BBN: {
+      double complex yc[4], fc[4];
+      for (int j = 0; j < 4; ++j) yc[j] = y[j];
+      bbn_eval(yc, t, p, fc);
+      for (int j = 0; j < 4; ++j) f[j] = creal(fc[j]);
+      break;
+    }
     case SYS_KAPS:
       for (int pr = 0; pr < s->n / 2; ++pr) {
         double ie = s->inv_eps[pr];
@@ -342,6 +398,9 @@ static void sys_jac(const osys* s, const double* y, double t, const double* p, d
     case SYS_DAMPED_OSC:
       J[1] = 1.0;
       J[2] = -p[0]; J[3] = -p[1];
+      break;
+    case SYS_BBN:
+      bbn_jac(y, t, p, J, dT);
       break;
     case SYS_KAPS:
       for (int pr = 0; pr < n / 2; ++pr) {

@@ -58,6 +58,7 @@ KIND = dict(
     linquad=7,
     damped_osc=8,
     kaps=9,
+    bbn=11,
 )
 
 
@@ -168,6 +169,13 @@ def linquad(n, off_rows, off_cols, off_vals, quad, off_pidx=None, n_p=None):
         off_pidx=np.full(len(rows), -1, np.int32) if off_pidx is None
         else np.asarray(off_pidx, np.int32),
     )
+
+
+def bbn():
+    """The four-species BBN network (examples/bbn_estimation/main.py:74-161),
+    x = Q/T the independent variable, parameters (log10 eta_10, N_eff);
+    derivatives by the complex step."""
+    return System("bbn", 4, 2)
 
 
 def kaps(n_pairs, epsilon_min):

@@ -528,3 +528,88 @@ def bordered_ensemble(size, n, seed=13):
     lam = -(10.0 ** rng.uniform(0.0, 4.0, size=(size, n)))
     y0 = rng.uniform(-1.0, 1.0, size=(size, n))
     return y0, lam
+
+
+# --------------------------------------------------------------------------
+# A four-species Big Bang nucleosynthesis network (n, p, D, 4He): a stiff,
+# non-autonomous right-hand side with exp, sqrt, a 1.5 power and a
+# comparison used as a 0/1 factor. The physics and constants are those of
+# the reference's consumer, examples/bbn_estimation/main.py:44-60 (constants)
+# and :74-161 (rates and network); the independent variable is x = Q/T.
+# --------------------------------------------------------------------------
+BBN_Q = 1.293            # MeV, n-p mass difference
+BBN_B_D = 2.225          # MeV, deuterium binding energy
+BBN_M_N = 938.272        # MeV, nucleon mass
+BBN_M_E = 0.511          # MeV, where g_star steps (e+e- annihilation)
+BBN_M_PL = 1.2209e22     # MeV, Planck mass
+BBN_ZETA3 = 1.2020569
+BBN_N_EFF_SM = 3.044
+BBN_TAU_N = 879.4 / 6.582119e-22   # neutron lifetime in MeV^-1
+BBN_SIGMA_NP = 4.0e-11   # MeV^-2, <sigma v> of n + p -> D
+BBN_SIGMA_DD = 3.4e-5    # MeV^-2, <sigma v> of D + D -> 4He
+BBN_X_SPAN = np.array([BBN_Q / 10.0, BBN_Q / 0.01])   # T from 10 MeV to 10 keV
+#: the consumer's prior box: log10(eta_10) in [0.5, 1.0], N_eff in [2, 4]
+#: (main.py:284-286), its observed abundances and errors (main.py:274-279),
+#: and its solver settings (main.py:176-196)
+BBN_PRIOR_LO = np.array([0.5, 2.0])
+BBN_PRIOR_HI = np.array([1.0, 4.0])
+BBN_OBSERVED = np.array([0.106, 4.2e-6])     # Y_P, D/H
+BBN_SIGMA = np.array([0.004, 0.3e-6])
+BBN_SOLVER = dict(lu_precision="fp32", rtol=1e-4, atol=1e-9, first_step=0.1, max_steps=256)
+
+
+def make_bbn(*, exp, sqrt):
+    """The network's dY/dx as a factory over the two functions a backend spells
+    its own way (the reference's dual-backend form, examples/dual_backend.py:
+    1-60): ``make_bbn(exp=math.exp, sqrt=math.sqrt)`` is a callback ``solve``
+    traces, ``make_bbn(exp=torch.exp, sqrt=torch.sqrt)`` evaluates it on
+    tensors. State Y = (Y_n, Y_p, Y_d, Y_4He), parameters (log10 eta_10,
+    N_eff). Y_n + Y_p + 2 Y_d + 4 Y_4He is conserved."""
+    pi = math.pi
+
+    def bbn(y, x, p):
+        eta = 10.0 ** (p[0] - 10.0)
+        temp = BBN_Q / x
+        # relativistic degrees of freedom, stepping down through e+e- annihilation
+        g_star = 3.91 + (10.75 - 3.91) * (temp > BBN_M_E) + (7.0 / 4.0) * (p[1] - BBN_N_EFF_SM)
+        hubble = sqrt(4.0 * pi ** 3 * g_star / 45.0) * temp ** 2 / BBN_M_PL
+        n_b = eta * (2.0 * BBN_ZETA3 / pi ** 2 * temp ** 3)
+        gamma_np = (255.0 / BBN_TAU_N) * (12.0 + 6.0 * x + x ** 2) / x ** 5 + 1.0 / BBN_TAU_N
+        gamma_pn = gamma_np * exp(-x)
+        # Saha equilibrium ratio Y_d / (Y_n Y_p)
+        k_d = n_b * (3.0 / 4.0) * (4.0 * pi / (BBN_M_N * temp)) ** 1.5 * exp(BBN_B_D / temp)
+        r_np = n_b * BBN_SIGMA_NP * (y[0] * y[1] - y[2] / k_d)
+        r_dd = n_b * BBN_SIGMA_DD * y[2] ** 2
+        den = hubble * x
+        return (
+            (gamma_pn * y[1] - gamma_np * y[0] - r_np) / den,
+            (gamma_np * y[0] - gamma_pn * y[1] - r_np) / den,
+            (r_np - 2.0 * r_dd) / den,
+            r_dd / den,
+        )
+
+    return bbn
+
+
+bbn = make_bbn(exp=math.exp, sqrt=math.sqrt)
+
+
+def bbn_initial_state():
+    """Weak equilibrium at x0 = Q / 10 MeV: Y_n / Y_p = e^-x0 (main.py:220-225)."""
+    x0 = BBN_X_SPAN[0]
+    return np.array([math.exp(-x0) / (1.0 + math.exp(-x0)), 1.0 / (1.0 + math.exp(-x0)),
+                     1e-20, 0.0])
+
+
+def bbn_grid(n):
+    """A near-square (log10 eta_10, N_eff) grid over the prior box, flattened
+    (the consumer's chi^2-grid benchmark, main.py:240-247)."""
+    side = int(math.ceil(math.sqrt(n)))
+    ee, nn = np.meshgrid(np.linspace(BBN_PRIOR_LO[0], BBN_PRIOR_HI[0], side),
+                         np.linspace(BBN_PRIOR_LO[1], BBN_PRIOR_HI[1], side))
+    return np.column_stack([ee.ravel(), nn.ravel()])[:n]
+
+
+def bbn_abundances(final):
+    """(Y_P, D/H) from final states (..., 4): Y_P = 4 Y_4He, D/H = Y_d / Y_p."""
+    return np.stack([4.0 * final[..., 3], final[..., 2] / final[..., 1]], axis=-1)

@@ -380,3 +380,45 @@ def test_callbacks_built_by_a_factory_of_math_operations_trace():
         y, p = rng.uniform(-1, 1, 2), rng.uniform(0.5, 2, 1)
         assert np.array_equal(cg.rhs_from_graph(a, y, 0.0, p), cg.rhs_from_graph(b, y, 0.0, p))
         np.testing.assert_allclose(cg.rhs_from_graph(a, y, 0.0, p), device(y, 0.0, p), rtol=1e-14)
+
+
+def test_a_comparison_used_as_a_number():
+    """``a + (b - a) * (x > c)``, the branchless select the reference's callbacks
+    write (examples/bbn_estimation/main.py:74-84): traced as select(cond, 1.0,
+    0.0) and then the same multiply and add as Python's bool arithmetic, so the
+    values are bit-identical on both sides of the step, signed zeros included;
+    its derivative is the other factor's."""
+    import math
+
+    from menodax_amd import _codegen as C
+
+    def f(y, t, p):
+        temp = 1.293 / t
+        return (3.91 + (10.75 - 3.91) * (temp > 0.511) * y[0], -y[1] * (t > 2.0),
+                (t < 1.0) - (t >= 1.0) + 0.5 * y[0])
+
+    traced = C.trace(f, 3, 0)
+    for t in (0.5, 1.0, 2.0, 2.5, 3.0):
+        y = [2.0, 3.0, 0.0]
+        got = C.rhs_from_graph(traced, y, t, [])
+        want = np.array(f(y, t, []), dtype=float)
+        assert np.array_equal(got, want) and all(
+            math.copysign(1.0, a) == math.copysign(1.0, b) for a, b in zip(got, want))
+        J, dT = C.jacobian_from_graph(traced, y, t, [])
+        assert J[0, 0] == 3.91 * 0.0 + (10.75 - 3.91) * (1.293 / t > 0.511)
+        assert J[1, 1] == -(1.0 if t > 2.0 else 0.0)
+        assert J[2, 0] == 0.5 and np.all(dT[:2] == 0.0)
+
+
+def test_the_bbn_network_traces_to_the_math_form():
+    """The dual-form BBN factory traced through the math module equals its plain
+    Python evaluation to rounding (integer powers are taken by squaring when
+    traced, as numba takes them, and by pow in Python)."""
+    from menodax_amd import _codegen as C
+
+    traced = C.trace(S.bbn, 4, 2)
+    y = S.bbn_initial_state()
+    for x in (0.2, 2.0, 2.6, 40.0):
+        for p in ([0.5, 2.0], [0.785, 3.044], [1.0, 4.0]):
+            np.testing.assert_allclose(C.rhs_from_graph(traced, y, x, p),
+                                       np.array(S.bbn(y, x, p)), rtol=1e-14, atol=0)

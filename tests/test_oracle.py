@@ -17,6 +17,8 @@ trajectories move by one step; the means agree to 2e-5 and the extremes to one
 step.
 """
 
+import math
+
 import numpy as np
 import pytest
 
@@ -306,3 +308,46 @@ def test_refined_fp32_robertson_dense():
     assert np.abs(hr - h64).max() < 5e-7
     with pytest.raises(ValueError, match="FP32"):
         O.rodas5p(O.robertson(), y0, times, p, lu_precision="fp64", lu_refine=True)
+
+
+# --- the BBN network: a non-polynomial, non-autonomous right-hand side ---------
+def test_bbn_oracle_conserves_baryon_number():
+    """Y_n + Y_p + 2 Y_d + 4 Y_4He = 1 is a linear invariant of the network, which
+    Rosenbrock methods keep to rounding with an exact Jacobian: the FP64 solve
+    holds it to 1e-12 over the prior box (a wrong sign or index in any rate
+    breaks it)."""
+    y0 = S.bbn_initial_state()
+    p = S.bbn_grid(16)
+    hist, _ = O.rodas5p(O.bbn(), np.tile(y0, (16, 1)), S.BBN_X_SPAN, p, lu_precision="fp64",
+                        rtol=1e-8, atol=1e-12, first_step=0.1)
+    np.testing.assert_allclose(hist @ np.array([1.0, 1.0, 2.0, 4.0]), 1.0, rtol=0, atol=1e-12)
+
+
+def test_bbn_oracle_reproduces_the_consumers_calibration():
+    """At Planck's eta_10 = 6.1 and N_eff = 3.044 the four-species network gives
+    Y_P ~ 0.106 and D/H ~ 4.2e-6: the values the reference's consumer takes as its
+    observations, calibrated to this model (examples/bbn_estimation/main.py:
+    268-279). A tight FP64 solve lands within 0.5% of both."""
+    y0 = S.bbn_initial_state()[None]
+    p = np.array([[math.log10(6.1), 3.044]])
+    hist, _ = O.rodas5p(O.bbn(), y0, S.BBN_X_SPAN, p, lu_precision="fp64", rtol=1e-10,
+                        atol=1e-14, first_step=0.1)
+    np.testing.assert_allclose(S.bbn_abundances(hist[0, -1]), S.BBN_OBSERVED, rtol=5e-3)
+
+
+@pytest.mark.parametrize("x", [0.15, 1.0, 2.5, 2.6, 8.0, 120.0])
+def test_bbn_oracle_derivatives_are_the_generated_ones(x):
+    """The oracle's complex-step Jacobian and df/dx (an independent method) equal
+    the code generator's forward-mode derivatives of the traced callback to
+    rounding, on both sides of the g_star step at x = Q/m_e ~ 2.53."""
+    from menodax_amd import _codegen as C
+
+    traced = C.trace(S.bbn, 4, 2)
+    y = np.array([0.3, 0.6, 1e-4, 0.02])
+    p = np.array([0.8, 3.3])
+    J, dT = O.jacobian(O.bbn(), y, x, p)
+    Jg, dTg = C.jacobian_from_graph(traced, y, x, p)
+    np.testing.assert_allclose(J, Jg, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(dT, dTg, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(O.rhs(O.bbn(), y, x, p), C.rhs_from_graph(traced, y, x, p),
+                               rtol=1e-14, atol=0)
