@@ -68,6 +68,12 @@ def device():
     return torch.device("cuda", torch.cuda.current_device())
 
 
+def n_cu():
+    """Compute units of the current device."""
+    torch = _torch()
+    return torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+
+
 def as_f64_device(x, dev):
     torch = _torch()
     if isinstance(x, torch.Tensor):

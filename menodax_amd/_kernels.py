@@ -182,6 +182,14 @@ def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None
     return defs
 
 
+def resident_groups(lds_per_group: int, n_cu: int) -> int:
+    """Workgroups the GPU keeps resident when each takes ``lds_per_group``
+    bytes of LDS: per compute unit as many as its LDS holds, at most one per
+    SIMD (these kernels' register use leaves a SIMD one wave)."""
+    per_cu = 4 if lds_per_group <= 0 else min(4, LDS_BYTES_PER_CU // lds_per_group)
+    return n_cu * per_cu
+
+
 def rodas5p_block(requested, n_aug: int, lu_precision: str, n_traj: int, n_cu: int,
                   placement=None, sparsity=None, lu_refine=False) -> int:
     """The workgroup size of a plain or joint Rodas5P solve: the caller's, else
@@ -189,9 +197,11 @@ def rodas5p_block(requested, n_aug: int, lu_precision: str, n_traj: int, n_cu: i
     step (``n_aug``, the joint vector's length, above ``UNROLLED_MAX_VARS``;
     LDS placement) cannot keep its eight stage vectors in LDS at 64 lanes:
     then the larger of 32 and 16 lanes at which the stage vectors, the stage
-    argument and the state all fit, provided every workgroup still has a SIMD
-    of its own (at most 4 per compute unit). One wave per SIMD either way, so
-    the narrower workgroup only moves the vectors out of scratch. The
+    argument and the state all fit, provided the whole ensemble is resident at
+    once at that size -- a narrow workgroup's LDS share limits how many a
+    compute unit holds (128 FP32 joint rows at 16 lanes: 96 KiB, one per CU,
+    4,096 lanes on 256 CUs, where 64-lane workgroups with the vectors in
+    scratch keep 16,384), so a larger ensemble stays at 64 lanes. The
     reference's stiff Van der Pol rings, 1,000 trajectories, FP32 sparse LU:
     the 128-variable ring 5.26 -> 3.61 ms, the joint solve of the 64-variable
     ring with one parameter direction 7.82 -> 5.48 ms; the 64-variable ring's
@@ -206,24 +216,31 @@ def rodas5p_block(requested, n_aug: int, lu_precision: str, n_traj: int, n_cu: i
     if stage_vectors * DEFAULT_BLOCK <= LDS_BYTES_PER_CU:
         return DEFAULT_BLOCK
     for block in (32, 16):
-        if ((stage_vectors + 16 * n_aug) * block <= LDS_BYTES_PER_CU
-                and -(-n_traj // block) <= 4 * n_cu):
+        group = (stage_vectors + 16 * n_aug) * block
+        if group <= LDS_BYTES_PER_CU and -(-n_traj // block) <= resident_groups(group, n_cu):
             return block
     return DEFAULT_BLOCK
 
 
-def tsit5_block(requested, n_vars: int) -> int:
+def tsit5_block(requested, n_vars: int, n_traj: int, n_cu: int) -> int:
     """The workgroup size of a Tsit5 solve: the caller's, else a wavefront,
-    except that a rolled step (more than ``UNROLLED_MAX_VARS`` variables)
-    takes the largest of 64, 32 and 16 lanes at which its seven stage vectors
-    fit in LDS. The reference's non-stiff Van der Pol rings, 1,000
-    trajectories: 64 variables 0.81 ms at 64 lanes (stage vectors in scratch),
-    0.55 ms at 32; 128 variables 1.60 ms at 64 lanes, 1.05 ms at 16."""
+    except that a rolled step (more than ``UNROLLED_MAX_VARS`` variables) of
+    an ensemble that is resident at once at the narrower size takes the
+    largest of 64, 32 and 16 lanes at which its seven stage vectors fit in LDS
+    (the residency bound as for ``rodas5p_block``: a 128-variable ring at 16
+    lanes takes 144 KiB, one workgroup per CU). The reference's non-stiff Van
+    der Pol rings, 1,000 trajectories: 64 variables 0.81 ms at 64 lanes (stage
+    vectors in scratch), 0.55 ms at 32; 128 variables 1.60 ms at 64 lanes,
+    1.05 ms at 16."""
     if requested is not None or n_vars <= UNROLLED_MAX_VARS:
         return trajectories_per_block_or_default(requested)
     for b in (64, 32, 16):
-        if tsit5_placement(n_vars, b)["MDX_LDS_K"]:
+        place = tsit5_placement(n_vars, b)
+        if not place["MDX_LDS_K"]:
+            continue
+        if b == DEFAULT_BLOCK or -(-n_traj // b) <= resident_groups(place["lds_bytes"], n_cu):
             return b
+        break
     return DEFAULT_BLOCK
 
 

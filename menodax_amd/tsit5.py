@@ -93,7 +93,7 @@ def solve(
         pcoeff=float(pcoeff),
         icoeff=float(icoeff),
         dcoeff=float(dcoeff),
-        trajectories_per_block=_kernels.tsit5_block(trajectories_per_block, n_vars),
+        trajectories_per_block=_block(trajectories_per_block, n_vars, n),
         unit_weights=error_weights is None,
         tableau_in_registers=_tableau(tableau_in_registers, n_vars, n, None, None),
     )
@@ -221,6 +221,14 @@ def solve_sensitivities(ode_fn, y0, t_span, params, *, wrt_y0=True, wrt_params=T
     return (state, sens, stats) if return_stats else (state, sens)
 
 
+def _block(requested, n_vars, n_traj):
+    """``_kernels.tsit5_block`` for this device (its compute units are only
+    asked for when the rule needs them: a rolled step and no caller's size)."""
+    if requested is not None or n_vars <= _kernels.UNROLLED_MAX_VARS:
+        return _kernels.tsit5_block(requested, n_vars, n_traj, 0)
+    return _kernels.tsit5_block(requested, n_vars, n_traj, _common.n_cu())
+
+
 def _traced(ode_fn, y0, t_span, params, *, rtol=1e-8, atol=1e-10, first_step=None,
             max_steps=100000, return_stats=False, error_weights=None, pcoeff=0.0, icoeff=1.0,
             dcoeff=0.0, backend="auto", sens_error_control=True, sens_param_columns=None,
@@ -234,8 +242,12 @@ def _traced(ode_fn, y0, t_span, params, *, rtol=1e-8, atol=1e-10, first_step=Non
     if hasattr(y0, "shape") and len(y0.shape) in (1, 2):
         _check_backend(y0.shape[-1], backend)
     n_vars = y0.shape[-1] if hasattr(y0, "shape") and len(y0.shape) in (1, 2) else 0
+    n_traj = 1
+    for x in (y0, params):
+        if hasattr(x, "shape") and len(x.shape) == 2:
+            n_traj = max(n_traj, int(x.shape[0]))
     options = (("pcoeff", float(pcoeff)), ("icoeff", float(icoeff)), ("dcoeff", float(dcoeff)),
-               ("trajectories_per_block", _kernels.tsit5_block(trajectories_per_block, n_vars)),
+               ("trajectories_per_block", _block(trajectories_per_block, n_vars, n_traj)),
                ("unit_weights", True))
     return _ops.traced_solve("tsit5", ode_fn, y0, t_span, params, options=options,
                              sparsity=None, tableau=tableau_in_registers, rtol=rtol, atol=atol,

@@ -229,8 +229,14 @@ def test_the_tableau_goes_to_registers_for_small_systems_and_ensembles():
 
 def test_a_rolled_tsit5_step_takes_the_largest_block_its_stage_vectors_fit():
     B = _kernels.tsit5_block
-    assert B(None, 3) == 64 and B(None, 24) == 64 and B(8, 64) == 8
-    assert B(None, 64) == 32 and B(None, 128) == 16 and B(None, 400) == 64
+    assert B(None, 3, 1000, 256) == 64 and B(None, 24, 1000, 256) == 64
+    assert B(8, 64, 1000, 256) == 8
+    assert B(None, 64, 1000, 256) == 32 and B(None, 128, 1000, 256) == 16
+    assert B(None, 400, 1000, 256) == 64
+    # a large ensemble is not resident at the narrow size: one 16-lane group of
+    # the 128-variable ring per CU is 4,096 lanes, fewer than 64-lane groups keep
+    assert B(None, 128, 100000, 256) == 64 and B(None, 64, 100000, 256) == 64
+    assert B(None, 128, 4096, 256) == 16 and B(None, 128, 4097, 256) == 64
     assert _kernels.tsit5_placement(64, 32)["MDX_LDS_K"] == 1
     assert _kernels.tsit5_placement(64, 64)["MDX_LDS_K"] == 0
 
@@ -247,7 +253,12 @@ def test_a_small_sparse_ensemble_narrows_its_workgroup_to_keep_stage_vectors_in_
     assert B(None, 128, "fp32", 1000, 256, None, None) == 64
     assert B(None, 128, "fp32", 1000, 256, "rolled", ring) == 64
     assert B(8, 128, "fp32", 1000, 256, None, ring) == 8
-    # more workgroups than SIMDs, or nothing fits at 16 lanes: a wavefront
+    # the ensemble not resident at once at the narrow size (a 16-lane group of
+    # 128 FP32 joint rows takes 96 KiB: one per CU), or nothing fits at 16
+    # lanes: a wavefront
+    assert B(None, 128, "fp32", 10000, 256, None, ring) == 64
+    assert B(None, 128, "fp32", 4096, 256, None, ring) == 16
+    assert B(None, 128, "fp32", 4097, 256, None, ring) == 64
     assert B(None, 128, "fp32", 20000, 256, None, ring) == 64
     assert B(None, 440, "fp32", 1000, 256, None, ring) == 64
     place = _kernels.rodas5p_placement(128, "fp32", 16, "lds")

@@ -6,10 +6,13 @@ The mode has no reference counterpart; what pins it:
     solution (tests/test_oracle.py). The kernel forms J x with the generated
     forward sweep and the oracle with its closed-form Jacobian summed in column
     order, so the residuals may differ in their last double bits; the residual
-    is rounded to float before its solve, which absorbs nearly all of that.
-    The bar: every step counter equal and histories within 1e-10 (absolute,
-    on states of order one), with the share of bit-identical trajectories
-    printed;
+    is rounded to float before its solve, which absorbs nearly all of that,
+    but now and then a last-bit difference flips an accept/reject decision
+    (1 trajectory in 2,000 on Robertson). The bar: at least 99% of the
+    trajectories take exactly the oracle's step counts and agree with it to
+    1e-10 (absolute, on states of order one); every trajectory lies within ten
+    times the solve's own tolerance (atol + rtol |y|) of the oracle. The
+    shares of equal counters and of bit-identical trajectories are printed;
   - bit-identity across the kernel's own placements and launch shapes (the
     same arithmetic in the same order);
   - the property the mode exists for: the history follows the FP64-LU
@@ -29,13 +32,15 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-def _close(tag, hist, st, ref, rst):
-    for k in rst:
-        np.testing.assert_array_equal(np.asarray(st[k]), rst[k], err_msg=f"{tag} {k}")
-    same = np.all(hist == ref, axis=tuple(range(1, hist.ndim)))
-    print(f"{tag}: {same.mean() * 100:.1f}% of trajectories bit-identical to the oracle, "
-          f"max abs {np.abs(hist - ref).max():.2e}", flush=True)
-    np.testing.assert_allclose(hist, ref, rtol=0, atol=TOL)
+def _close(tag, hist, st, ref, rst, rtol=1e-6, atol=1e-8):
+    steps = ((np.asarray(st["accepted_steps"]) == rst["accepted_steps"])
+             & (np.asarray(st["rejected_steps"]) == rst["rejected_steps"]))
+    bit = np.all(hist == ref, axis=tuple(range(1, hist.ndim)))
+    print(f"{tag}: {steps.mean() * 100:.2f}% same step counts, {bit.mean() * 100:.1f}% "
+          f"bit-identical to the oracle, max abs {np.abs(hist - ref).max():.2e}", flush=True)
+    assert steps.mean() >= 0.99
+    np.testing.assert_allclose(hist[steps], ref[steps], rtol=0, atol=TOL)
+    assert np.all(np.abs(hist - ref) <= 10.0 * (atol + rtol * np.abs(ref)))
 
 
 @pytest.mark.parametrize("placement", [None, "rolled"])
