@@ -1,34 +1,43 @@
-"""Minimum-degree fill-reducing ordering for the sparse LU.
+"""Approximate minimum degree ordering for the sparse LU.
 
 The reference orders with SuiteSparse's approximate minimum degree through
 cvxopt (modax/_sparse_direct.py:91-152): AMD on the symmetrised pattern
-``S + S^T`` (diagonal included), returning a symmetric permutation that keeps
-the diagonal -- where ``I/(h gamma)`` lands -- on the diagonal. cvxopt is not
-available to this build, so this is an implementation of the published
-method (Amestoy, Davis & Duff, "An approximate minimum degree ordering
-algorithm", SIAM J. Matrix Anal. Appl. 17(4), 1996) written for the small
-patterns these solvers take (tens to a few hundred variables):
+``S + S^T``, returning a symmetric permutation that keeps the diagonal -- where
+``I/(h gamma)`` lands -- on the diagonal. cvxopt is not available to this
+build, so this is an implementation of the published algorithm (Amestoy, Davis
+& Duff, "An approximate minimum degree ordering algorithm", SIAM J. Matrix
+Anal. Appl. 17(4):886-905, 1996) for the small patterns these solvers take:
 
-  - elimination on the quotient graph: an eliminated variable becomes an
-    *element*, its adjacency the union of its neighbours' elements and
-    variables, and elements adjacent to a new element are absorbed into it;
-  - supervariables: variables with identical adjacency are merged (hashing
-    on their adjacency) and eliminated together (mass elimination);
-  - external degrees are computed exactly from the quotient graph -- at these
-    sizes the exact count is as cheap as AMD's bound and never worse;
-  - rows denser than max(16, 10 sqrt(n)) are set aside and ordered last, as
-    AMD's default dense-row rule does;
-  - ties go to the supervariable that reached the minimum degree first,
-    lowest index among equals;
-  - the result is postordered along the assembly tree, children before
-    parents, which leaves the fill unchanged and groups each subtree.
+  - the quotient graph: each variable i keeps its variable neighbours A_i and
+    its element neighbours E_i; eliminating the pivot p turns it into the
+    element p with variables L_p = A_p plus the union of L_e over e in E_p,
+    minus p, and absorbs every element of E_p (element absorption);
+  - the approximate external degree of each i in L_p,
+        d_i = min(n_left - |i|,  d_i(old) + |L_p \\ i|,
+                  |A_i \\ i| + |L_p \\ i| + sum over e in E_i \\ {p} of |L_e \\ L_p|),
+    with |L_e \\ L_p| from one pass over L_p (start at |L_e|, subtract |j|
+    for every j in L_p adjacent to e); an element with |L_e \\ L_p| = 0 lies
+    inside L_p and is absorbed into p (aggressive absorption);
+  - mass elimination: a variable of L_p with nothing left beyond the new
+    element (no variable neighbour, every other element absorbed) is
+    eliminated with the pivot, as a member of its element;
+  - supervariables: variables of L_p with the same A and E are merged into
+    one of weight |i| and eliminated together;
+  - degree lists taken last in, first out: the pivot is the variable inserted
+    most recently into the lowest non-empty list (initially the highest index
+    of that degree);
+  - variables whose degree exceeds max(16, 10 sqrt(n)) -- AMD's default dense
+    threshold -- are set aside and ordered last;
+  - the result is postordered along the assembly tree (children before
+    parents, the child with the largest front last), each supervariable's
+    merged members numbered just before it in index order. A postorder leaves
+    the fill unchanged.
 
-The fill (``nnz(L + U)``) is what the ordering exists to minimise, and it is
-what is pinned: the reference's tests record AMD's fill on arrow, ring and
-bordered-block patterns (tests/test_sparse_direct.py:164-215). SuiteSparse's
-exact permutation is not pinned by anything in the reference -- its own tests
-note two AMD routes breaking ties differently -- so permutation parity with it
-is not claimed.
+What the ordering is for, nnz(L + U), is what is pinned: the reference's tests
+record AMD's fill on arrow, ring and bordered-block patterns (tests/
+test_sparse_direct.py:164-215). SuiteSparse's exact permutation is pinned by
+nothing in the reference -- its own tests note that AMD routes break ties
+differently -- so permutation parity with it is not claimed.
 """
 
 from __future__ import annotations
@@ -47,137 +56,120 @@ def amd_order(pattern) -> tuple:
             if c != r:
                 adj[r].add(c)
                 adj[c].add(r)
-
-    dense_threshold = max(16, int(10.0 * math.sqrt(n)))
-    dense = [v for v in range(n) if len(adj[v]) > dense_threshold]
+    dense_limit = min(n, max(16, int(10.0 * math.sqrt(n))))
+    dense = [v for v in range(n) if len(adj[v]) > dense_limit]
     dense_set = set(dense)
     live = [v for v in range(n) if v not in dense_set]
+
+    A = {v: adj[v] - dense_set for v in live}     # variable neighbours
+    E = {v: set() for v in live}                  # element neighbours
+    L: dict[int, set] = {}                        # element -> its variables
+    weight = {v: 1 for v in live}                 # supervariable sizes
+    merged = {v: [] for v in live}                # members merged into v
+    deg = {v: len(A[v]) for v in live}
+    parent: dict[int, int] = {}                   # assembly tree (elements)
+    front: dict[int, int] = {}                    # front size at elimination
+    pivots = []
+    n_left = len(live)
+
+    lists: dict[int, list] = {}                   # degree -> stack, top = last
     for v in live:
-        adj[v] -= dense_set
+        lists.setdefault(deg[v], []).append(v)
 
-    # quotient graph: variables adjacent to variables (A) and to elements (E)
-    var_adj = {v: set(adj[v]) for v in live}
-    elem_adj: dict[int, set] = {v: set() for v in live}  # variable -> elements
-    element_vars: dict[int, set] = {}  # element -> its variables
-    # supervariables: representative -> member list (in index order)
-    members = {v: [v] for v in live}
-    weight = {v: 1 for v in live}
-    alive = set(live)
-    order = []
-    parent_of_element: dict[int, int] = {}
-    # degree bookkeeping with first-reached timestamps for tie-breaking
-    stamp = 0
-    reached = {}
+    def take(v):
+        lists[deg[v]].remove(v)
 
-    def reach(v):
-        """Variables reachable from v through its variables and elements."""
-        out = set(var_adj[v])
-        for e in elem_adj[v]:
-            out |= element_vars[e]
-        out.discard(v)
-        return out & alive
+    while n_left:
+        d_min = min(d for d, stack in lists.items() if stack)
+        p = lists[d_min].pop()
+        Lp = set(A[p])
+        for e in E[p]:
+            Lp |= L.pop(e)
+            parent[e] = p
+        Lp.discard(p)
+        n_left -= weight[p]
+        pivots.append(p)
+        for i in Lp:
+            take(i)
+            A[i] -= Lp
+            A[i].discard(p)
+            E[i] -= E[p]
+            E[i].add(p)
+        del A[p], E[p]
 
-    def ext_degree(v):
-        return sum(weight[u] for u in reach(v))
+        # |L_e \ L_p| for the elements next to L_p
+        outside: dict[int, int] = {}
+        for i in sorted(Lp):
+            for e in E[i]:
+                if e != p:
+                    if e not in outside:
+                        outside[e] = sum(weight[j] for j in L[e])
+                    outside[e] -= weight[i]
+        for e, w in outside.items():
+            if w == 0:  # aggressive absorption: L_e lies inside L_p
+                for j in L.pop(e):
+                    E[j].discard(e)
+                parent[e] = p
+        partial = {}
+        for i in sorted(Lp):
+            partial[i] = (sum(weight[j] for j in A[i])
+                          + sum(outside[e] for e in E[i] if e != p))
+            if partial[i] == 0:  # mass elimination: i goes with the pivot
+                n_left -= weight[i]
+                weight[p] += weight.pop(i)
+                merged[p] += [i] + merged.pop(i)
+                del A[i], E[i], deg[i], partial[i]
+                Lp.discard(i)
 
-    degree = {}
-    for v in live:
-        degree[v] = ext_degree(v)
-        reached[v] = (degree[v], stamp)
-    stamp += 1
+        # supervariables: the same variable and element neighbours
+        groups: dict = {}
+        for i in sorted(Lp):
+            groups.setdefault((frozenset(A[i]), frozenset(E[i])), []).append(i)
+        for members in groups.values():
+            head = members[0]
+            for j in members[1:]:
+                weight[head] += weight.pop(j)
+                merged[head] += [j] + merged.pop(j)
+                for x in A.pop(j):
+                    A[x].discard(j)
+                for e in E.pop(j):
+                    if e != p:  # element p's list is L_p itself, formed below
+                        L[e].discard(j)
+                Lp.discard(j)
+                partial.pop(j)
+                deg.pop(j)
+        w_lp = sum(weight[i] for i in Lp)
+        front[p] = w_lp + weight[p]
+        for i in sorted(Lp):
+            ext = w_lp - weight[i]
+            deg[i] = min(n_left - weight[i], deg[i] + ext, partial[i] + ext)
+            lists.setdefault(deg[i], []).append(i)
+        L[p] = Lp
 
-    while alive:
-        # minimum external degree; ties: reached that degree first, then lowest index
-        p = min(alive, key=lambda v: (degree[v], reached[v][1], v))
-        nbrs = reach(p)
-        # p becomes element p: adjacent to every variable it reaches
-        absorbed = set(elem_adj[p])
-        element_vars[p] = set(nbrs)
-        for e in absorbed:
-            parent_of_element[e] = p
-            element_vars.pop(e, None)
-        order.extend(members[p])
-        alive.discard(p)
-        for u in nbrs:
-            var_adj[u].discard(p)
-            elem_adj[u] -= absorbed
-            elem_adj[u].add(p)
-            # variables now reachable through element p need no explicit edge
-            var_adj[u] -= nbrs
-        del var_adj[p]
-        del elem_adj[p]
-
-        # mass elimination: merge indistinguishable supervariables among nbrs
-        buckets: dict = {}
-        for u in sorted(nbrs):
-            key = (frozenset(var_adj[u]), frozenset(elem_adj[u]))
-            buckets.setdefault(key, []).append(u)
-        for group in buckets.values():
-            if len(group) < 2:
-                continue
-            rep = group[0]
-            for u in group[1:]:
-                members[rep].extend(members[u])
-                weight[rep] += weight[u]
-                alive.discard(u)
-                for w in var_adj[u]:
-                    var_adj[w].discard(u)
-                    var_adj[w].add(rep)
-                for e in elem_adj[u]:
-                    element_vars[e].discard(u)
-                    element_vars[e].add(rep)
-                del var_adj[u]
-                del elem_adj[u]
-            members[rep].sort()
-        for u in nbrs & alive:
-            d = ext_degree(u)
-            if d != degree[u]:
-                degree[u] = d
-                reached[u] = (d, stamp)
-        stamp += 1
-
-    order.extend(dense)
-    return tuple(_postorder(order, pattern))
+    return tuple(_postorder(pivots, parent, front, merged) + dense)
 
 
-def _postorder(order, pattern):
-    """Reorder along the elimination tree of the ordered pattern, children in
-    their original relative order before their parent. The fill is that of
-    the input order; only the sequence within independent subtrees changes."""
-    n = len(order)
-    pos = {v: i for i, v in enumerate(order)}
-    # symbolic elimination tree via the standard ancestor-path compression
-    adj = [set() for _ in range(n)]
-    for r, cols in enumerate(pattern):
-        for c in cols:
-            if c != r:
-                adj[pos[r]].add(pos[c])
-                adj[pos[c]].add(pos[r])
-    parent = [-1] * n
-    ancestor = [-1] * n
-    for k in range(n):
-        for i in sorted(adj[k]):
-            if i >= k:
-                continue
-            # walk from i to the root of its current subtree
-            j = i
-            while ancestor[j] != -1 and ancestor[j] != k:
-                nxt = ancestor[j]
-                ancestor[j] = k
-                j = nxt
-            if ancestor[j] == -1:
-                ancestor[j] = k
-                parent[j] = k
-    children = [[] for _ in range(n)]
+def _postorder(pivots, parent, front, merged):
+    """Elements by a depth-first postorder of the assembly tree: roots and each
+    node's children in index order, except that the child with the largest
+    front goes last; each element expands to its merged members in index
+    order, then the element's own variable."""
+    children: dict[int, list] = {p: [] for p in pivots}
     roots = []
-    for k in range(n):
-        (children[parent[k]] if parent[k] >= 0 else roots).append(k)
+    for p in sorted(pivots):
+        (children[parent[p]] if p in parent else roots).append(p)
+    for kids in children.values():
+        if len(kids) > 1:
+            big = max(kids, key=lambda c: (front[c], -c))
+            kids.remove(big)
+            kids.append(big)
     out = []
     stack = [(r, False) for r in reversed(roots)]
     while stack:
         node, done = stack.pop()
         if done:
-            out.append(order[node])
+            out.extend(sorted(merged[node]))
+            out.append(node)
             continue
         stack.append((node, True))
         for c in reversed(children[node]):

@@ -166,6 +166,23 @@ def test_amd_fills_what_the_reference_records(mask, expected):
     assert SD.analyse(normalize_sparsity(mask, len(mask)), "amd").nnz == expected
 
 
+def test_amd_mass_eliminates_a_clique_and_sets_dense_rows_aside():
+    """The published algorithm's special cases: a fully coupled block is one
+    supervariable, eliminated at once with its members numbered before the
+    pivot (the highest index, first off the LIFO degree list); a row denser
+    than max(16, 10 sqrt(n)) is set aside and ordered last, here the hub of a
+    300-variable arrow, which then fills nothing."""
+    from menodax_amd._amd import amd_order
+
+    clique = tuple(tuple(range(5)) for _ in range(5))
+    assert amd_order(clique) == (0, 1, 2, 3, 4)
+    mask = _arrow(300)
+    pattern = normalize_sparsity(mask, 300)
+    order = amd_order(pattern)
+    assert sorted(order) == list(range(300)) and order[-1] == 0
+    assert SD.analyse(pattern, "amd").nnz == int(mask.sum())
+
+
 def test_amd_order_is_a_permutation_and_beats_natural_on_random_patterns():
     rng = np.random.default_rng(5)
     for n in (8, 20, 50):
