@@ -23,15 +23,20 @@ Anal. Appl. 17(4):886-905, 1996) for the small patterns these solvers take:
     eliminated with the pivot, as a member of its element;
   - supervariables: variables of L_p with the same A and E are merged into
     one of weight |i| and eliminated together;
-  - degree lists taken last in, first out: the pivot is the variable inserted
-    most recently into the lowest non-empty list (initially the highest index
-    of that degree);
+  - degree lists taken first in, first out: the pivot is the variable that
+    has held the lowest degree longest (a variable whose degree an update
+    leaves unchanged keeps its place; initially the lowest index of that
+    degree). SuiteSparse takes its lists last in, first out; on rings and
+    chains that sweeps around the graph and builds an elimination tree about
+    twice as tall (a 64-variable Van der Pol ring: triangular solves 33
+    dependent rows deep instead of 19, chain 63 instead of 34) for the same
+    fill, and a solve that runs on one lane per trajectory pays for every level;
   - variables whose degree exceeds max(16, 10 sqrt(n)) -- AMD's default dense
     threshold -- are set aside and ordered last;
   - the result is postordered along the assembly tree (children before
-    parents, the child with the largest front last), each supervariable's
-    merged members numbered just before it in index order. A postorder leaves
-    the fill unchanged.
+    parents, the child with the largest front last), each element's variable
+    followed by the variables merged into it, in index order. A postorder
+    leaves the fill unchanged.
 
 What the ordering is for, nnz(L + U), is what is pinned: the reference's tests
 record AMD's fill on arrow, ring and bordered-block patterns (tests/
@@ -72,16 +77,17 @@ def amd_order(pattern) -> tuple:
     pivots = []
     n_left = len(live)
 
-    lists: dict[int, list] = {}                   # degree -> stack, top = last
+    lists: dict[int, dict] = {}                   # degree -> queue (insertion order)
     for v in live:
-        lists.setdefault(deg[v], []).append(v)
+        lists.setdefault(deg[v], {})[v] = None
 
     def take(v):
-        lists[deg[v]].remove(v)
+        del lists[deg[v]][v]
 
     while n_left:
-        d_min = min(d for d, stack in lists.items() if stack)
-        p = lists[d_min].pop()
+        d_min = min(d for d, queue in lists.items() if queue)
+        p = next(iter(lists[d_min]))
+        del lists[d_min][p]
         Lp = set(A[p])
         for e in E[p]:
             Lp |= L.pop(e)
@@ -90,7 +96,6 @@ def amd_order(pattern) -> tuple:
         n_left -= weight[p]
         pivots.append(p)
         for i in Lp:
-            take(i)
             A[i] -= Lp
             A[i].discard(p)
             E[i] -= E[p]
@@ -115,6 +120,7 @@ def amd_order(pattern) -> tuple:
             partial[i] = (sum(weight[j] for j in A[i])
                           + sum(outside[e] for e in E[i] if e != p))
             if partial[i] == 0:  # mass elimination: i goes with the pivot
+                take(i)
                 n_left -= weight[i]
                 weight[p] += weight.pop(i)
                 merged[p] += [i] + merged.pop(i)
@@ -137,13 +143,17 @@ def amd_order(pattern) -> tuple:
                         L[e].discard(j)
                 Lp.discard(j)
                 partial.pop(j)
+                take(j)
                 deg.pop(j)
         w_lp = sum(weight[i] for i in Lp)
         front[p] = w_lp + weight[p]
         for i in sorted(Lp):
             ext = w_lp - weight[i]
-            deg[i] = min(n_left - weight[i], deg[i] + ext, partial[i] + ext)
-            lists.setdefault(deg[i], []).append(i)
+            d = min(n_left - weight[i], deg[i] + ext, partial[i] + ext)
+            if d != deg[i]:
+                take(i)
+                deg[i] = d
+                lists.setdefault(d, {})[i] = None
         L[p] = Lp
 
     return tuple(_postorder(pivots, parent, front, merged) + dense)
@@ -152,8 +162,8 @@ def amd_order(pattern) -> tuple:
 def _postorder(pivots, parent, front, merged):
     """Elements by a depth-first postorder of the assembly tree: roots and each
     node's children in index order, except that the child with the largest
-    front goes last; each element expands to its merged members in index
-    order, then the element's own variable."""
+    front goes last; each element expands to its own variable, then the
+    variables merged into it in index order."""
     children: dict[int, list] = {p: [] for p in pivots}
     roots = []
     for p in sorted(pivots):
@@ -168,8 +178,8 @@ def _postorder(pivots, parent, front, merged):
     while stack:
         node, done = stack.pop()
         if done:
-            out.extend(sorted(merged[node]))
             out.append(node)
+            out.extend(sorted(merged[node]))
             continue
         stack.append((node, True))
         for c in reversed(children[node]):
