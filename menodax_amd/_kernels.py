@@ -182,6 +182,44 @@ def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None
     return defs
 
 
+def adjoint_lds_bytes(n_vars: int, n_lu: int, lu_precision: str) -> int:
+    """Per-lane LDS of the adjoint's LDS placement: the eight stage vectors (in
+    the LU's type), their eight adjoints and the factors' double copy."""
+    lu_bytes = 4 if lu_precision == "fp32" else 8
+    return 8 * n_vars * lu_bytes + 8 * n_vars * 8 + 8 * n_lu
+
+
+def lu_size(n_vars: int, sparsity=None, ordering: str = "amd") -> int:
+    """Stored elements of the iteration matrix: n^2 dense, else the sparse
+    factorisation's L + U."""
+    if sparsity is None:
+        return n_vars * n_vars
+    from . import _sparse_direct
+
+    return _sparse_direct.sparse_direct_solver_for(sparsity, ordering).compressed.size
+
+
+def adjoint_block(n_vars: int, n_lu: int, lu_precision: str, n_traj: int, n_cu: int):
+    """The rolled Rodas5P adjoint's (placement, workgroup size) for an
+    ensemble of ``n_traj``: the LDS placement at the widest of 64, 32 and 16
+    lanes whose LDS share holds the stage vectors, their adjoints and the
+    factors with the whole ensemble resident at once, else the scratch
+    ("rolled") step at 64 lanes. Residency decides because the reverse step
+    waits on every access to these arrays: a small ensemble leaves SIMDs idle
+    and gains from LDS latency, while a large one at a narrow LDS workgroup
+    keeps fewer lanes resident than 64-lane scratch workgroups. The 20-variable
+    Van der Pol chain (67 factor elements), 8,192 trajectories: FP64 LU
+    405 -> 219 ms (LDS at 32 lanes), FP32 369 -> 224 ms (64 lanes); 65,536
+    trajectories stay in scratch (907 / 786 ms against 908-1366 / 794-813 ms
+    for the LDS forms)."""
+    per_lane = adjoint_lds_bytes(n_vars, n_lu, lu_precision)
+    for block in (64, 32, 16):
+        group = per_lane * block
+        if group <= LDS_BYTES_PER_CU and -(-n_traj // block) <= resident_groups(group, n_cu):
+            return "lds", block
+    return "rolled", DEFAULT_BLOCK
+
+
 def resident_groups(lds_per_group: int, n_cu: int) -> int:
     """Workgroups the GPU keeps resident when each takes ``lds_per_group``
     bytes of LDS: per compute unit as many as its LDS holds, at most one per
@@ -350,8 +388,10 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
     """Generate the full Rodas5P kernel source; returns (source, details).
     ``adjoint=True`` generates the discrete-adjoint kernel instead (the
     reverse-mode contractions of the callback; registers up to
-    ``ADJOINT_UNROLLED_MAX_VARS`` variables, the rolled scratch step above,
-    never LDS)."""
+    ``ADJOINT_UNROLLED_MAX_VARS`` variables, the rolled step above; with
+    ``placement="lds"`` the rolled adjoint keeps its stage vectors, their
+    adjoints and the factors in LDS when they fit at the workgroup size, else
+    in scratch)."""
     if options.lu_precision not in ("fp32", "fp64"):
         raise ValueError(f"lu_precision must be 'fp32' or 'fp64', got {options.lu_precision!r}")
     if options.lu_refine:
@@ -408,12 +448,18 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
     if adjoint and spec is not None:
         raise ValueError("the adjoint kernel differentiates the plain solve; drop the sensitivity spec")
     placement = options.placement
+    adjoint_lds = False
     if adjoint:
         # the adjoint reads the state from per-lane arrays: registers for a few
-        # variables (every stage loop unrolled), the rolled scratch step above
+        # variables (every stage loop unrolled), the rolled step above, its
+        # stage-indexed arrays in scratch or, when they fit, in LDS
         if placement is None:
             placement = "registers" if n_aug <= ADJOINT_UNROLLED_MAX_VARS else "rolled"
-        elif placement in ("lds", WAVE):
+        elif placement == WAVE:
+            placement = "rolled"
+        elif placement == "lds":
+            per_lane = adjoint_lds_bytes(n_vars, structure.size, options.lu_precision)
+            adjoint_lds = per_lane * block <= LDS_BYTES_PER_CU
             placement = "rolled"
     # a dense matrix above the unrolled size lives in per-lane memory: LDS
     # when it fits (the forward step only)
@@ -421,6 +467,9 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
                                        and not adjoint) else 0)
     place = rodas5p_placement(n_aug, options.lu_precision, block, placement, dense_matrix,
                               k_bytes=8 if options.lu_refine else None)
+    if adjoint_lds:
+        place = dict(place, MDX_LDS_K=1, placement="lds",
+                     lds_bytes=adjoint_lds_bytes(n_vars, structure.size, options.lu_precision) * block)
     defs = {
         "MDX_N": n_vars,
         "MDX_NP": n_params,
@@ -448,6 +497,7 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
         "MDX_TAB_REGS": 1 if (place["MDX_ROLLED"] == 0 and options.tableau_in_registers
                               and not adjoint and spec is None) else 0,
         "MDX_REFINE": 1 if options.lu_refine else 0,
+        "MDX_ADJ_LDS": 1 if adjoint_lds else 0,
     }
     sens_src = ""
     if options.save_hook is not None or not options.save_history:

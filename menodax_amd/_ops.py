@@ -43,6 +43,7 @@ class _Solver:
     options: object        # KernelOptions or Tsit5Options, tableau choice unresolved
     tableau: object        # the caller's tableau_in_registers (None: size rule)
     ckpt_every: int
+    block: object = None   # the caller's trajectories_per_block (None: size rules)
 
     def forward(self, n, wave_ok=False):
         """The forward kernels for an ensemble of ``n``; ``wave_ok`` lets the
@@ -61,10 +62,14 @@ class _Solver:
             return _kernels.rodas5p_kernel(self.ode_fn, self.n_vars, self.n_params, opts)
         return _kernels.tsit5_kernel(self.ode_fn, self.n_vars, self.n_params, opts)
 
-    def adjoint(self):
+    def adjoint(self, n):
+        """The adjoint kernel for an ensemble of ``n``."""
         if self.method == "rodas5p":
-            return _kernels.rodas5p_adjoint_kernel(self.ode_fn, self.n_vars, self.n_params,
-                                                   self.options)
+            from .rodas5P import _adjoint_options
+
+            return _kernels.rodas5p_adjoint_kernel(
+                self.ode_fn, self.n_vars, self.n_params,
+                _adjoint_options(self.options, self.n_vars, n, self.block))
         return _kernels.tsit5_adjoint_kernel(self.ode_fn, self.n_vars, self.n_params,
                                              self.options)
 
@@ -79,9 +84,11 @@ def solver_key(method, ode_fn, n_vars, n_params, sparsity, options, tableau, ckp
     kw = dict(options)
     if sparsity is not None:
         kw["sparsity"] = normalize_sparsity(sparsity, int(n_vars))
+    block = kw.get("trajectories_per_block")
+    kw["trajectories_per_block"] = _kernels.trajectories_per_block_or_default(block)
     opts = (_kernels.KernelOptions if method == "rodas5p" else _kernels.Tsit5Options)(**kw)
     solver = _Solver(method, ode_fn, int(n_vars), int(n_params), opts, tableau,
-                     int(ckpt_every))
+                     int(ckpt_every), block)
     key = _INDEX.get(solver)
     if key is None:
         key = len(_SOLVERS)
@@ -129,7 +136,7 @@ def ensemble_adjoint(g_hist: torch.Tensor, y0: torch.Tensor, t_span: torch.Tenso
     s = _SOLVERS[solver]
     n = y0.shape[0]
     job = _autodiff.AdjointJob(
-        built=s.forward(n), adjoint=s.adjoint(), times=t_span.contiguous(), n=n,
+        built=s.forward(n), adjoint=s.adjoint(n), times=t_span.contiguous(), n=n,
         n_vars=s.n_vars, n_params=s.n_params, weights=None,
         launch_kw=dict(dt0=dt0, rtol=rtol, atol=atol, max_steps=max_steps, refill=None,
                        stream=None, order=None),

@@ -346,14 +346,15 @@ def _solve_t_source(layout, unrolled: bool) -> str:
     """The transposed solve ``M^T x = b`` with the same factors (M = L U in
     elimination order): ``U^T z = b`` forward, each row's pivot scaled by the
     stored inverse diagonal and its upper entries scattered onward, then
-    ``L^T x = z`` backward, scattering the lower entries. For the adjoint."""
+    ``L^T x = z`` backward, scattering the lower entries. For the adjoint; the
+    factors are read through any indexable type (a register array, or the
+    adjoint's per-lane LDS rows)."""
     n = layout.n_vars
     row_ptr, diag_ptr = layout.row_ptr, layout.diag_ptr
     origin, col_origin = layout.row_origin, layout.col_origin
     if unrolled:
-        lines = ["template <class T>",
-                 "__device__ __forceinline__ void sparse_solve_t(const T* __restrict__ lu, "
-                 "T* __restrict__ r) {"]
+        lines = ["template <class T, class L>",
+                 "__device__ __forceinline__ void sparse_solve_t(const L& lu, T* __restrict__ r) {"]
         for i in range(n):
             row, piv = int(origin[i]), int(diag_ptr[i])
             body = "".join(f" r[{int(col_origin[p])}] = mdx::fmaT(-lu[{p}], z, r[{int(col_origin[p])}]);"
@@ -372,8 +373,8 @@ def _solve_t_source(layout, unrolled: bool) -> str:
     tabs, (ROW, DIAG_, ORIG, CORIG) = _pack(row_ptr, diag_ptr, origin, col_origin)
     return "\n".join([
         _const_table("SOLVE_T_TAB", tabs),
-        "template <class T>",
-        "__device__ __forceinline__ void sparse_solve_t(const T* __restrict__ lu, T* __restrict__ r) {",
+        "template <class T, class L>",
+        "__device__ __forceinline__ void sparse_solve_t(const L& lu, T* __restrict__ r) {",
         f"  for (int i = 0; i < {n}; ++i) {{",
         f"    const int row = SOLVE_T_TAB[{ORIG} + i];",
         f"    const int pivot = SOLVE_T_TAB[{DIAG_} + i];",

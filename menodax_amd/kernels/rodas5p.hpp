@@ -570,8 +570,8 @@ __device__ __forceinline__ void dense_solve(const L& lu, const Pivots& ipiv,
 
 // M^T x = b for the factorisation above (PM = LU, swaps in ipiv): U^T z = b,
 // L^T w = z, then the swaps undone in reverse order. Used by the adjoint.
-template <class T>
-__device__ __forceinline__ void dense_solve_t(const T* __restrict__ lu, const Pivots& ipiv,
+template <class T, class L>
+__device__ __forceinline__ void dense_solve_t(const L& lu, const Pivots& ipiv,
                                               T* __restrict__ r) {
 #pragma unroll 1
   for (int i = 0; i < N; ++i) {
@@ -809,9 +809,10 @@ __device__ __forceinline__ void solve(const L& lu, const Pivots& ipiv, LUT* __re
 }
 
 #if MDX_ADJOINT
-// M^T x = b against the step's factorisation, in double (the adjoint).
-__device__ __forceinline__ void solve_t(const double* __restrict__ lu, const Pivots& ipiv,
-                                        double* __restrict__ r) {
+// M^T x = b against the step's factorisation, in double (the adjoint); the
+// factors through any indexable type holding doubles.
+template <class L>
+__device__ __forceinline__ void solve_t(const L& lu, const Pivots& ipiv, double* __restrict__ r) {
 #if MDX_SPARSE
   (void)ipiv;
   sys::sparse_solve_t(lu, r);

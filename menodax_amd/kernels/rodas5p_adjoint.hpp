@@ -33,6 +33,13 @@
 //   for the two end stages)
 // Workspace, per lane, [slot][lane] (coalesced): max_ckpt checkpoints of
 // N + 8 doubles, then ckpt_every step records of N + 4 doubles.
+//
+// With MDX_ADJ_LDS=1 (rolled step only) the per-lane arrays the reverse step
+// indexes by stage -- the stage vectors k (the forward step's own LDS stage
+// vectors, free while a step is reversed), their adjoints k_bar and the double
+// copy of the factors -- live in LDS, [element][lane], instead of scratch: a
+// lane whose ensemble leaves most SIMDs idle waits on each access, and LDS
+// answers in a fraction of scratch's time. Same arithmetic, same bits.
 #pragma once
 
 namespace mdx {
@@ -43,6 +50,34 @@ namespace r5 {
 #define MDX_ADJ_STAGES _Pragma("unroll 1")
 #else
 #define MDX_ADJ_STAGES _Pragma("unroll")
+#endif
+
+#ifndef MDX_ADJ_LDS
+#define MDX_ADJ_LDS 0
+#endif
+#if MDX_ADJ_LDS && !(MDX_ROLLED && MDX_LDS_K && !MDX_LDS_U && !MDX_LDS_Y && !MDX_LDS_LU)
+#error "the adjoint's LDS arrays go with the rolled step's LDS stage vectors only"
+#endif
+
+#if MDX_ADJ_LDS
+__shared__ double lds_kb[8 * N * MDX_BLOCK];
+__shared__ double lds_lud[MDX_NLU * MDX_BLOCK];
+// element e of this lane's row in an [element][lane] LDS array
+struct LdsRows {
+  double* base;
+  __device__ __forceinline__ double& operator[](int e) const { return base[e * MDX_BLOCK]; }
+};
+// Stage vectors k[q][j] and adjoints k_bar[q][j]; each user re-reads the lane
+// index (MDX_LANE) where its loop starts, as the forward step does.
+struct KStore {};
+#define KV(q, j) KQ(q, j)
+#define KBV(q, j) lds_kb[((q) * N + (j)) * MDX_BLOCK + lane_]
+#else
+struct KStore {
+  LUT v[8][N];
+};
+#define KV(q, j) ks.v[q][j]
+#define KBV(q, j) kbar[q][j]
 #endif
 
 constexpr int CK = N + 8;   // checkpoint words
@@ -94,7 +129,8 @@ __device__ __forceinline__ double step_size(const State& s) {
 // The step's factorisation and stage vectors at (y, t, h), by the forward
 // step's arithmetic.
 __device__ void recompute(const double* __restrict__ y, double t, double h, PA& p,
-                          LUT* __restrict__ lu, Pivots& ipiv, LUT (*__restrict__ k)[N]) {
+                          LUT* __restrict__ lu, Pivots& ipiv, KStore& ks) {
+  (void)ks;
   const double inv_dt = 1.0 / h;
   const double t_end = t + h;
   const double dtgamma_inv = 1.0 / (h * GAMMA);
@@ -108,19 +144,20 @@ __device__ void recompute(const double* __restrict__ y, double t, double h, PA& 
   double u[N];
   MDX_ADJ_STAGES
   for (int S = 0; S < 8; ++S) {
+    MDX_LANE();
     double ts;
     if (S == 0) {
       for (int j = 0; j < N; ++j) u[j] = y[j];
       ts = t;
     } else if (S < 6) {
       for (int j = 0; j < N; ++j) {
-        double acc = AM[S - 1][0] * (double)k[0][j];
-        for (int q = 1; q < S; ++q) acc = fmaT(AM[S - 1][q], (double)k[q][j], acc);
+        double acc = AM[S - 1][0] * (double)KV(0, j);
+        for (int q = 1; q < S; ++q) acc = fmaT(AM[S - 1][q], (double)KV(q, j), acc);
         u[j] = y[j] + acc;
       }
       ts = S < 5 ? t + CT[S - 1] * h : t_end;
     } else {
-      for (int j = 0; j < N; ++j) u[j] = u[j] + (double)k[S - 1][j];
+      for (int j = 0; j < N; ++j) u[j] = u[j] + (double)KV(S - 1, j);
       ts = t_end;
     }
     double work[N];
@@ -131,33 +168,34 @@ __device__ void recompute(const double* __restrict__ y, double t, double h, PA& 
       if (S == 0) {
         v = fmaT(h * DV[0], DTJ(j), work[j]);
       } else {
-        double acc = CM[S][0] * (double)k[0][j];
-        for (int q = 1; q < S; ++q) acc = fmaT(CM[S][q], (double)k[q][j], acc);
+        double acc = CM[S][0] * (double)KV(0, j);
+        for (int q = 1; q < S; ++q) acc = fmaT(CM[S][q], (double)KV(q, j), acc);
         v = S < 5 ? fmaT(acc, inv_dt, fmaT(h * DV[S], DTJ(j), work[j])) : fmaT(acc, inv_dt, work[j]);
       }
       r[j] = (LUT)v;
     }
     solve(lu, ipiv, r);
-    for (int j = 0; j < N; ++j) k[S][j] = r[j];
+    for (int j = 0; j < N; ++j) KV(S, j) = r[j];
   }
 }
 
 // u_S of the step (the argument of stage S's right-hand side) and its time.
 __device__ __forceinline__ double stage_point(int S, const double* __restrict__ y, double t,
-                                              double h, LUT (*__restrict__ k)[N],
-                                              double* __restrict__ u) {
+                                              double h, KStore& ks, double* __restrict__ u) {
+  (void)ks;
+  MDX_LANE();
   if (S == 0) {
     for (int j = 0; j < N; ++j) u[j] = y[j];
     return t;
   }
   const int A = S < 6 ? S - 1 : 4;
   for (int j = 0; j < N; ++j) {
-    double acc = AM[A][0] * (double)k[0][j];
-    for (int q = 1; q <= A; ++q) acc = fmaT(AM[A][q], (double)k[q][j], acc);
+    double acc = AM[A][0] * (double)KV(0, j);
+    for (int q = 1; q <= A; ++q) acc = fmaT(AM[A][q], (double)KV(q, j), acc);
     u[j] = y[j] + acc;
   }
-  if (S >= 6) for (int j = 0; j < N; ++j) u[j] = u[j] + (double)k[5][j];
-  if (S >= 7) for (int j = 0; j < N; ++j) u[j] = u[j] + (double)k[6][j];
+  if (S >= 6) for (int j = 0; j < N; ++j) u[j] = u[j] + (double)KV(5, j);
+  if (S >= 7) for (int j = 0; j < N; ++j) u[j] = u[j] + (double)KV(6, j);
   return S == 0 ? t : (S < 5 ? t + CT[S - 1] * h : t + h);
 }
 
@@ -168,20 +206,26 @@ __device__ void reverse_step(const AdjointArgs& g, i64 i, PA& p, const double* _
   const SolveArgs& a = g.s;
   LUT lu[MDX_NLU];
   Pivots ipiv;
-  LUT k[8][N];
-  recompute(y, t, h, p, lu, ipiv, k);
+  KStore ks;
+  recompute(y, t, h, p, lu, ipiv, ks);
+  MDX_LANE();
+#if MDX_ADJ_LDS
+  const LdsRows lud{&lds_lud[lane_]};
+#else
   double lud[MDX_NLU];
+  double kbar[8][N];
+#endif
   for (int q = 0; q < MDX_NLU; ++q) lud[q] = (double)lu[q];
   const double inv_dt = 1.0 / h;
 
-  double kbar[8][N], ybar[N], ucarry[N], dTbar[N];
+  double ybar[N], ucarry[N], dTbar[N];
   for (int q = 0; q < 8; ++q)
-    for (int j = 0; j < N; ++j) kbar[q][j] = 0.0;
+    for (int j = 0; j < N; ++j) KBV(q, j) = 0.0;
   for (int j = 0; j < N; ++j) {
     ybar[j] = 0.0;
     dTbar[j] = 0.0;
     ucarry[j] = lam[j];   // y_{n+1} = u7 + k7
-    kbar[7][j] = lam[j];
+    KBV(7, j) = lam[j];
   }
   // saves the step produced: theta1 y + theta (y_new + theta1 (h1 + theta (h2 + theta h3)))
   for (int sv = save_lo; sv < save_hi; ++sv) {
@@ -192,16 +236,17 @@ __device__ void reverse_step(const AdjointArgs& g, i64 i, PA& p, const double* _
       const double gj = gbar_at(g, i, sv, j);
       ybar[j] += theta1 * gj;
       ucarry[j] += theta * gj;
-      kbar[7][j] += theta * gj;
-      for (int q = 0; q < 8; ++q) kbar[q][j] += (H1[q] * c1 + H2[q] * c2 + H3[q] * c3) * gj;
+      KBV(7, j) += theta * gj;
+      for (int q = 0; q < 8; ++q) KBV(q, j) += (H1[q] * c1 + H2[q] * c2 + H3[q] * c3) * gj;
     }
   }
   MDX_ADJ_STAGES
   for (int S = 7; S >= 0; --S) {
+    MDX_LANE();
     double rb[N], kd[N], u[N], ub[N];
     for (int j = 0; j < N; ++j) {
-      rb[j] = kbar[S][j];
-      kd[j] = (double)k[S][j];
+      rb[j] = KBV(S, j);
+      kd[j] = (double)KV(S, j);
       ub[j] = 0.0;
     }
     solve_t(lud, ipiv, rb);
@@ -212,9 +257,9 @@ __device__ void reverse_step(const AdjointArgs& g, i64 i, PA& p, const double* _
     }
     for (int q = 0; q < S; ++q) {
       const double c = CM[S][q] * inv_dt;
-      for (int j = 0; j < N; ++j) kbar[q][j] += c * rb[j];
+      for (int j = 0; j < N; ++j) KBV(q, j) += c * rb[j];
     }
-    const double ts = stage_point(S, y, t, h, k, u);
+    const double ts = stage_point(S, y, t, h, ks, u);
     sys::f_vjp(u, ts, p, rb, ub, pbar);
     if (S >= 5) {
       for (int j = 0; j < N; ++j) ub[j] += ucarry[j];
@@ -223,13 +268,13 @@ __device__ void reverse_step(const AdjointArgs& g, i64 i, PA& p, const double* _
       // u_S = u_{S-1} + k_{S-1}
       for (int j = 0; j < N; ++j) {
         ucarry[j] = ub[j];
-        kbar[S - 1][j] += ub[j];
+        KBV(S - 1, j) += ub[j];
       }
     } else if (S > 0) {
       // u_S = y + sum_q A_{S,q} k_q
       for (int j = 0; j < N; ++j) ybar[j] += ub[j];
       for (int q = 0; q < S; ++q)
-        for (int j = 0; j < N; ++j) kbar[q][j] += AM[S - 1][q] * ub[j];
+        for (int j = 0; j < N; ++j) KBV(q, j) += AM[S - 1][q] * ub[j];
     } else {
       for (int j = 0; j < N; ++j) ybar[j] += ub[j];
     }

@@ -15,6 +15,7 @@ import pytest
 
 from menodax_amd import _codegen as cg
 from menodax_amd import _kernels, _runtime
+from menodax_amd._sparsity import normalize_sparsity
 from tests import systems as S
 from oracle import oracle as O
 
@@ -263,6 +264,38 @@ def test_a_small_sparse_ensemble_narrows_its_workgroup_to_keep_stage_vectors_in_
     assert B(None, 440, "fp32", 1000, 256, None, ring) == 64
     place = _kernels.rodas5p_placement(128, "fp32", 16, "lds")
     assert place["MDX_LDS_K"] == place["MDX_LDS_U"] == place["MDX_LDS_Y"] == 1
+
+
+def test_the_rolled_adjoint_keeps_its_stage_arrays_in_lds_while_the_ensemble_is_resident():
+    A = _kernels.adjoint_block
+    # 20-variable chain, 67 factor elements: 3,096 B a lane in FP64, 2,188 in FP32
+    assert _kernels.adjoint_lds_bytes(20, 67, "fp64") == 8 * 20 * 8 * 2 + 8 * 67
+    assert _kernels.adjoint_lds_bytes(20, 67, "fp32") == 8 * 20 * 4 + 8 * 20 * 8 + 8 * 67
+    assert A(20, 67, "fp64", 8192, 256) == ("lds", 32)
+    assert A(20, 67, "fp32", 8192, 256) == ("lds", 64)
+    # 8,193 FP64 trajectories need 257 groups of 32 (one 99 KB group a CU), then
+    # 16 lanes: three groups a CU
+    assert A(20, 67, "fp64", 8193, 256) == ("lds", 16)
+    assert A(20, 67, "fp64", 12288, 256) == ("lds", 16)
+    assert A(20, 67, "fp64", 12289, 256) == ("rolled", 64)
+    assert A(20, 67, "fp64", 65536, 256) == ("rolled", 64)
+    assert A(20, 67, "fp32", 65536, 256) == ("rolled", 64)
+    # nothing fits at 16 lanes
+    assert A(200, 2000, "fp64", 10, 256) == ("rolled", 64)
+    fn, _ = S.vdp_lattice(10, mu=1.0, d=0.1, wrap=False)
+    sp = normalize_sparsity(S.vdp_sparsity(10, wrap=False), 20)
+    assert _kernels.lu_size(20, sp) == 67 and _kernels.lu_size(20) == 400
+    lds, d = _kernels.rodas5p_source(
+        fn, 20, 1, _kernels.KernelOptions(lu_precision="fp64", sparsity=sp, placement="lds",
+                                          trajectories_per_block=32), adjoint=True)
+    assert "#define MDX_ADJ_LDS 1" in lds and "#define MDX_LDS_K 1" in lds
+    assert d["placement"]["lds_bytes"] == 3096 * 32
+    assert _runtime.rtc_compile(lds, "chain_adjoint_lds.hip")[:4] == b"\x7fELF"
+    # at 64 lanes the FP64 arrays miss LDS: the scratch step
+    wide, _ = _kernels.rodas5p_source(
+        fn, 20, 1, _kernels.KernelOptions(lu_precision="fp64", sparsity=sp, placement="lds"),
+        adjoint=True)
+    assert "#define MDX_ADJ_LDS 0" in wide and "#define MDX_LDS_K 0" in wide
 
 
 def test_the_wave_placement_is_for_small_ensembles_of_dense_systems():

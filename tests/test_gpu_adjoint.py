@@ -1,7 +1,7 @@
 """The discrete adjoint of Rodas5P on the GPU.
 
-Pinned as the scope table asks: against central differences of the loss
-through the plain GPU solve (h = 1e-6 relative, 1e-5 relative agreement),
+Pinned against central differences of the loss through the plain GPU solve
+(h = 1e-6 relative, 1e-5 relative agreement),
 against the forward-sensitivity gradient (the reference's own derivative,
 _sensitivity.py, whose oracle restatement is bit-exact with the GPU in
 test_gpu_sensitivity.py) and against a closed form; and for its own
@@ -11,7 +11,8 @@ machinery, gradients independent of the checkpoint interval.
 import numpy as np
 import pytest
 
-from menodax_amd import rodas5P
+from menodax_amd import _kernels, rodas5P
+from menodax_amd._sparsity import normalize_sparsity
 from tests import systems as S
 
 pytestmark = pytest.mark.gpu
@@ -104,6 +105,42 @@ def test_adjoint_placements_give_the_same_bits(lu):
     assert np.all(np.isfinite(ref[0])) and np.all(np.isfinite(ref[1]))
     np.testing.assert_array_equal(got[0], ref[0])
     np.testing.assert_array_equal(got[1], ref[1])
+
+
+@pytest.mark.parametrize("lu,block", [("fp64", 32), ("fp64", 16), ("fp32", 64)])
+def test_the_chain_adjoint_in_lds_gives_the_scratch_kernels_bits(lu, block):
+    # the 20-variable chain's rolled adjoint with its stage vectors, their
+    # adjoints and the factors in LDS, against the same step in scratch; 96
+    # trajectories leave the last workgroup partly empty
+    import torch
+
+    fn, y0, mu, t, target, kw = _vdp_problem(10, 96, seed=8)
+    kw = dict(kw, lu_precision=lu, sparsity=S.vdp_sparsity(10, wrap=False), rtol=1e-8,
+              atol=1e-10)
+    hist = rodas5P.solve(fn, y0, t, mu, **kw)
+    _, g = _loss_and_cotangent(hist, target)
+    ref = rodas5P.adjoint_gradients(fn, y0, t, mu, g, ckpt_every=16, placement="rolled",
+                                    trajectories_per_block=64, **kw)
+    got = rodas5P.adjoint_gradients(fn, y0, t, mu, g, ckpt_every=16, placement="lds",
+                                    trajectories_per_block=block, **kw)
+    assert np.all(np.isfinite(ref[0])) and np.all(np.isfinite(ref[1]))
+    np.testing.assert_array_equal(got[0], ref[0])
+    np.testing.assert_array_equal(got[1], ref[1])
+    # by default a small ensemble takes the LDS placement, through either entry
+    opts = rodas5P._adjoint_options(
+        _kernels.KernelOptions(lu_precision=lu, sparsity=normalize_sparsity(kw["sparsity"], 20)),
+        20, 96, None)
+    assert opts.placement == "lds" and opts.trajectories_per_block == (32 if lu == "fp64" else 64)
+    dflt = rodas5P.adjoint_gradients(fn, y0, t, mu, g, ckpt_every=16, **kw)
+    np.testing.assert_array_equal(dflt[0], ref[0])
+    np.testing.assert_array_equal(dflt[1], ref[1])
+    dev = torch.device("cuda")
+    y0t = torch.tensor(y0, device=dev, requires_grad=True)
+    mut = torch.tensor(mu, device=dev, requires_grad=True)
+    h = rodas5P.solve(fn, y0t, t, mut, adjoint=True, ckpt_every=16, **kw)
+    (h * torch.tensor(g, device=dev)).sum().backward()
+    np.testing.assert_array_equal(y0t.grad.cpu().numpy(), ref[0])
+    np.testing.assert_array_equal(mut.grad.cpu().numpy(), ref[1])
 
 
 def test_stiff_robertson_fp32_lu_adjoint_against_sensitivities():
