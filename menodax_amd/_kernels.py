@@ -172,7 +172,10 @@ def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None
     if placement == "lds":
         lu_bytes = 4 if lu_precision == "fp32" else 8
         for macro, per_lane in (("MDX_LDS_LU", dense_matrix * lu_bytes),
-                                ("MDX_LDS_K", 8 * n_vars * (k_bytes or lu_bytes)),
+                                # refined solves keep each stage's right-hand
+                                # side in LDS beside the stage vectors
+                                ("MDX_LDS_K", 8 * n_vars * (k_bytes or lu_bytes)
+                                 + (8 * n_vars if k_bytes else 0)),
                                 ("MDX_LDS_U", 8 * n_vars), ("MDX_LDS_Y", 8 * n_vars)):
             if per_lane and lds + per_lane * block <= LDS_BYTES_PER_CU:
                 defs[macro] = 1
@@ -244,18 +247,27 @@ def rodas5p_block(requested, n_aug: int, lu_precision: str, n_traj: int, n_cu: i
     the 128-variable ring 5.26 -> 3.61 ms, the joint solve of the 64-variable
     ring with one parameter direction 7.82 -> 5.48 ms; the 64-variable ring's
     plain solve (stage vectors in LDS at 64 lanes) keeps 64 lanes: 1.31 ms
-    there, 1.41-1.45 ms at 32 or 16."""
+    there, 1.41-1.45 ms at 32 or 16.
+
+    Refined FP32 solves (``lu_refine``) take the narrow workgroup whatever the
+    ensemble size: their FP64 stage vectors and each stage's right-hand side,
+    which the residual's Jacobian-vector product reads back, cost more in
+    scratch than the lanes a narrow workgroup leaves unresident. The 100,000
+    uneven stiff trajectories (50 variables): 1.80 s at 32 lanes (one
+    workgroup a CU), 2.29 s at 16, 3.41 s at 64 with the stage vectors in
+    scratch."""
     if requested is not None:
         return trajectories_per_block_or_default(requested)
     if placement not in (None, "lds") or sparsity is None or n_aug <= UNROLLED_MAX_VARS:
         return DEFAULT_BLOCK
     lu_bytes = 4 if lu_precision == "fp32" and not lu_refine else 8
-    stage_vectors = 8 * n_aug * lu_bytes
+    stage_vectors = 8 * n_aug * lu_bytes + (8 * n_aug if lu_refine else 0)
     if stage_vectors * DEFAULT_BLOCK <= LDS_BYTES_PER_CU:
         return DEFAULT_BLOCK
     for block in (32, 16):
         group = (stage_vectors + 16 * n_aug) * block
-        if group <= LDS_BYTES_PER_CU and -(-n_traj // block) <= resident_groups(group, n_cu):
+        if group <= LDS_BYTES_PER_CU and (
+                lu_refine or -(-n_traj // block) <= resident_groups(group, n_cu)):
             return block
     return DEFAULT_BLOCK
 

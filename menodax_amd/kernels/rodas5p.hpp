@@ -240,6 +240,10 @@ __shared__ KT lds_k[8 * NA * MDX_LSTRIDE];
 #else
 #define KQ(q, j) k[q][j]
 #endif
+#if MDX_REFINE && MDX_LDS_K
+// a refined stage's right-hand side in double, [element][lane]
+__shared__ double lds_b[N * MDX_LSTRIDE];
+#endif
 #if MDX_LDS_Y
 __shared__ double lds_y[NA * MDX_LSTRIDE];
 #define YJ(st, j) lds_y[(j) * MDX_LSTRIDE + lane_]
@@ -914,10 +918,10 @@ struct StageVec {
 // The refinement's right-hand side as J x is produced: element j of J x lands
 // in r[j] = (LUT)(fma(-x_j, 1/(h gamma), b_j) + (J x)_j), the residual
 // b - M x against M = I/(h gamma) - J in double, x_j = ks(S, j).
-template <class KS>
+template <class KS, class B>
 struct Residual {
   LUT* r;
-  const double* b;
+  B b;
   KS ks;
   int S;
   double gi;
@@ -938,12 +942,11 @@ struct Residual {
 // residual of b (the stage's right-hand side in double) against the double
 // iteration matrix at the step's base point is solved with the same FP32
 // factors and added, so the stored stage vector is x + dx in double.
-template <class YB, class KS, class L>
+template <class YB, class B, class KS, class L>
 __device__ __forceinline__ void refine(int S, const YB& ybase, double t, const PA& p, double gi,
-                                       const double* __restrict__ b, KS ks, const L& lu,
-                                       const Pivots& ipiv) {
+                                       B b, KS ks, const L& lu, const Pivots& ipiv) {
   LUT r[N];
-  sys::jvp(ybase, t, p, StageVec<KS>{ks, S}, Residual<KS>{r, b, ks, S, gi});
+  sys::jvp(ybase, t, p, StageVec<KS>{ks, S}, Residual<KS, B>{r, b, ks, S, gi});
   solve(lu, ipiv, r);
 #pragma unroll
   for (int j = 0; j < N; ++j) ks(S, j) = ks(S, j) + (double)r[j];
@@ -1028,7 +1031,8 @@ __device__ __forceinline__ void stage(const double* __restrict__ work,
 #pragma unroll
   for (int j = 0; j < N; ++j) k[S][j] = r[j];
 #if MDX_REFINE
-  refine(S, ybase, t, p, gi, b, [&](int q, int j) -> KT& { return k[q][j]; }, lu, ipiv);
+  refine(S, ybase, t, p, gi, (const double*)b, [&](int q, int j) -> KT& { return k[q][j]; }, lu,
+         ipiv);
 #endif
 }
 
@@ -1175,7 +1179,13 @@ __device__ __forceinline__ void rolled_stage(int S, const Z& z, double ts, const
   double work[N];
   sys::rhs(z, ts, p, work);
   LUT r[N];
-#if MDX_REFINE
+#if MDX_REFINE && MDX_LDS_K
+  // the right-hand side b, kept in LDS for the residual while the solve and
+  // the Jacobian-vector product hold the registers
+  MDX_LANE();
+  stage_rows(S, 0, work, dT, kst, dt_use, inv_dt,
+             [&](int i, double v) { r[i] = (LUT)v; lds_b[i * MDX_LSTRIDE + lane_] = v; });
+#elif MDX_REFINE
   // the right-hand side replaces f in work, element by element (each row's f
   // is read before its right-hand side is written), for the residual
   stage_rows(S, 0, work, dT, kst, dt_use, inv_dt,
@@ -1187,8 +1197,10 @@ __device__ __forceinline__ void rolled_stage(int S, const Z& z, double ts, const
   solve(lu, ipiv, r);
 #pragma unroll
   for (int j = 0; j < N; ++j) ks(S, j) = r[j];
-#if MDX_REFINE
-  refine(S, ybase, t, p, gi, work, ks, lu, ipiv);
+#if MDX_REFINE && MDX_LDS_K
+  refine(S, ybase, t, p, gi, LdsVec{&lds_b[lane_]}, ks, lu, ipiv);
+#elif MDX_REFINE
+  refine(S, ybase, t, p, gi, (const double*)work, ks, lu, ipiv);
 #endif
 #if MDX_NSENS > 0
 #pragma unroll MDX_SENS_UNROLL

@@ -262,6 +262,14 @@ def test_a_small_sparse_ensemble_narrows_its_workgroup_to_keep_stage_vectors_in_
     assert B(None, 128, "fp32", 4097, 256, None, ring) == 64
     assert B(None, 128, "fp32", 20000, 256, None, ring) == 64
     assert B(None, 440, "fp32", 1000, 256, None, ring) == 64
+    # refined solves: FP64 stage vectors plus the stage right-hand side, 3,600 B
+    # a lane at 50 variables, narrow whatever the ensemble size
+    assert B(None, 50, "fp32", 100000, 256, None, ring, True) == 32
+    assert B(None, 50, "fp32", 100000, 256, None, ring) == 64
+    assert B(None, 30, "fp32", 100000, 256, None, ring, True) == 64
+    assert B(None, 100, "fp32", 100000, 256, None, ring, True) == 16
+    refined = _kernels.rodas5p_placement(50, "fp32", 32, "lds", k_bytes=8)
+    assert refined["MDX_LDS_K"] == 1 and refined["lds_bytes"] == (3600 + 800) * 32
     place = _kernels.rodas5p_placement(128, "fp32", 16, "lds")
     assert place["MDX_LDS_K"] == place["MDX_LDS_U"] == place["MDX_LDS_Y"] == 1
 
