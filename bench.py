@@ -285,8 +285,26 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, err_sample=4096, tight_sample=
         built_r = _kernels.rodas5p_kernel(fn, n_vars, n_p, _kernels.KernelOptions(
             lu_precision="fp32", sparsity=sp, lu_refine=True, placement=best_r.placement,
             trajectories_per_block=best_r.trajectories_per_block))
-        timed("fp32_refined_refill", built_r, best_r.refill)
+        _, outs_r, ws_r, _ = timed("fp32_refined_refill", built_r, best_r.refill)
         res["fp32_refined_refill"]["tuned"] = refined_tune
+        if best_r.refill:
+            # the same first solve ranked by the pilot: about half of the refined
+            # solve is its slowest trajectory's own latency, which a late claim
+            # adds to the rest of the ensemble's
+            def run_refined_piloted():
+                return rodas5P.launch(built_r, y0_t, t_t, p_t, n, dt0=0.0, rtol=RTOL, atol=ATOL,
+                                      max_steps=100000, refill=True, outputs=outs_r,
+                                      workspace=ws_r, order="pilot")
+
+            o4, wall4, _ = time_solves(run_refined_piloted, 2, 0, torch)
+            ms4 = wall4 / 2 * 1e3
+            res["fp32_refined_refill_piloted"] = {
+                "ms": ms4, "trajectories_per_s": n / (ms4 / 1e3),
+                "order": f"a {rodas5P.PILOT_STEPS}-step pilot of every trajectory ranks them by "
+                         "steps left; pilot and sort inside the timed region",
+                "bit_identical": bool(np.array_equal(o4.ensemble_view().cpu().numpy(),
+                                                     hists["fp32_refined_refill"]))}
+            log_fn(f"uneven stiff fp32_refined_refill piloted: {ms4:.1f} ms")
     if not {"fp32_refill", "fp64_plain"} <= set(res):
         return {"variants": res}
     # the autotuner over placement x workgroup size on the whole ensemble, in
@@ -336,13 +354,19 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, err_sample=4096, tight_sample=
     fp64_ms = res["fp64_plain"]["ms"]
     target = {"criterion": "FP32-LU persistent refill >= 1.5x the plain FP64 one-lane kernel in "
                            "trajectories/s, with endpoint error vs the FP64 oracle <= 1e-6"}
-    for key in ("fp32_refill", "fp32_refined_refill"):
+    for key in ("fp32_refill", "fp32_refill_piloted", "fp32_refined_refill",
+                "fp32_refined_refill_piloted"):
         if key in res:
+            # a piloted solve is bit-identical to its unordered one (checked),
+            # so it has that one's error
+            base = key.replace("_piloted", "")
+            if base != key and not res[key]["bit_identical"]:
+                continue
             sp_ = fp64_ms / res[key]["ms"]
             target[key] = {"speedup_over_fp64_plain": sp_,
                            "max_abs_error_vs_fp64_oracle":
-                               err[key]["vs_fp64_oracle_same_tol"]["max_abs"],
-                           "met": bool(sp_ >= 1.5 and err[key]["target_met"])}
+                               err[base]["vs_fp64_oracle_same_tol"]["max_abs"],
+                           "met": bool(sp_ >= 1.5 and err[base]["target_met"])}
     return {
         "workload": "100,000 trajectories of dy/dt = A y + 0.01 y*y, 50 vars, A block arrow "
                     "(196 nnz) with per-trajectory stiffness up to 1e5, t in [0,10], 20 saves, "
