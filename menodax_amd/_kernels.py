@@ -202,6 +202,22 @@ def lu_size(n_vars: int, sparsity=None, ordering: str = "amd") -> int:
     return _sparse_direct.sparse_direct_solver_for(sparsity, ordering).compressed.size
 
 
+def adjoint_lds_supported(sparsity=None, ordering: str = "amd") -> bool:
+    """Whether the adjoint's LDS placement serves this pattern: dense systems
+    and sparse ones whose factorisation and solves are unrolled. With the
+    table-driven sparse routines (patterns above the unrolling thresholds) the
+    LDS adjoint kernel gave gradients different from the scratch kernel's, for
+    a cause not found (the emitted routines are clean under AddressSanitizer on
+    the host, the forward kernel with the same LDS layout is exact, and so is
+    the adjoint with either routine alone table driven), so those patterns run
+    the scratch adjoint."""
+    if sparsity is None:
+        return True
+    from . import _sparse_direct
+
+    return all(_sparse_direct.sparse_direct_solver_for(sparsity, ordering).unrolled())
+
+
 def adjoint_block(n_vars: int, n_lu: int, lu_precision: str, n_traj: int, n_cu: int):
     """The rolled Rodas5P adjoint's (placement, workgroup size) for an
     ensemble of ``n_traj``: the LDS placement at the widest of 64, 32 and 16
@@ -471,7 +487,8 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
             placement = "rolled"
         elif placement == "lds":
             per_lane = adjoint_lds_bytes(n_vars, structure.size, options.lu_precision)
-            adjoint_lds = per_lane * block <= LDS_BYTES_PER_CU
+            adjoint_lds = (per_lane * block <= LDS_BYTES_PER_CU
+                           and adjoint_lds_supported(options.sparsity, options.ordering))
             placement = "rolled"
     # a dense matrix above the unrolled size lives in per-lane memory: LDS
     # when it fits (the forward step only)

@@ -299,6 +299,20 @@ def test_the_rolled_adjoint_keeps_its_stage_arrays_in_lds_while_the_ensemble_is_
     assert "#define MDX_ADJ_LDS 1" in lds and "#define MDX_LDS_K 1" in lds
     assert d["placement"]["lds_bytes"] == 3096 * 32
     assert _runtime.rtc_compile(lds, "chain_adjoint_lds.hip")[:4] == b"\x7fELF"
+    # table-driven sparse routines keep the scratch adjoint
+    assert _kernels.adjoint_lds_supported(None) and _kernels.adjoint_lds_supported(sp)
+    from menodax_amd import _sparse_direct as SD
+
+    saved = SD.MAX_UNROLLED_UPDATES, SD.MAX_UNROLLED_SUBSTITUTIONS
+    SD.MAX_UNROLLED_UPDATES = SD.MAX_UNROLLED_SUBSTITUTIONS = 0
+    try:
+        assert not _kernels.adjoint_lds_supported(sp)
+        tab, _ = _kernels.rodas5p_source(
+            fn, 20, 1, _kernels.KernelOptions(lu_precision="fp64", sparsity=sp, placement="lds",
+                                              trajectories_per_block=32), adjoint=True)
+        assert "#define MDX_ADJ_LDS 0" in tab and "#define MDX_LDS_K 0" in tab
+    finally:
+        SD.MAX_UNROLLED_UPDATES, SD.MAX_UNROLLED_SUBSTITUTIONS = saved
     # at 64 lanes the FP64 arrays miss LDS: the scratch step
     wide, _ = _kernels.rodas5p_source(
         fn, 20, 1, _kernels.KernelOptions(lu_precision="fp64", sparsity=sp, placement="lds"),
