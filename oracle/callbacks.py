@@ -10,7 +10,10 @@ operation rounds once, as on the GPU). What the oracle then restates
 independently is the integration: the joint Rodas5P step, its
 block-triangular stage solve, the error norm over the augmented vector. The
 derivative graphs themselves are pinned apart from it -- against the
-closed-form Jacobians bit for bit and against central differences.
+closed-form Jacobians bit for bit, against central differences, and, for
+Robertson and a Van der Pol oscillator, against hand-derived closed-form
+sensitivity callbacks (``oracle.ClosedFormCallbacks``) pointwise and through
+a whole joint integration (tests/test_sensitivity.py).
 """
 
 from __future__ import annotations

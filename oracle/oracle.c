@@ -1266,6 +1266,109 @@ int oracle_jacobian(const int* sys_i, const double* sys_d, const int* off_rc,
   return 0;
 }
 
+/* Closed-form forward-sensitivity callbacks, derived by hand, for two systems
+ * the joint-solve tests use: Robertson (modax's stiff kinetics, three rates)
+ * and one Van der Pol oscillator y' = (v, mu p0 (1 - x^2) v - x) (vdp.py:
+ * 253-290 with one oscillator, omega 1, no coupling). They have the signatures
+ * of the callback solver's hooks (cb_f_t, cb_jac_t, cb_sens_t, cb_cpl_t), so
+ * the oracle's joint solve can run on them instead of on the derivative graphs
+ * the product's tracer emits -- an independent check of those graphs' second
+ * derivatives through a whole integration.
+ *
+ * Robertson: f = (-a y0 + b y1 y2, a y0 - b y1 y2 - c y1^2, c y1^2), p = (a, b, c)
+ *   J_y = [[-a, b y2, b y1], [a, -b y2 - 2c y1, -b y1], [0, 2c y1, 0]]
+ *   J_p = [[-y0, y1 y2, 0], [y0, -y1 y2, -y1^2], [0, 0, y1^2]]
+ *   D_y(J_y s)[v] = (b (v2 s1 + v1 s2), -b (v2 s1 + v1 s2) - 2c v1 s1, 2c v1 s1)
+ *   D_y(J_p q)[v] = (-v0 q0 + (v1 y2 + y1 v2) q1,
+ *                    v0 q0 - (v1 y2 + y1 v2) q1 - 2 y1 v1 q2, 2 y1 v1 q2) */
+void oracle_cf_robertson_f(const double* y, double t, const double* p, double* out) {
+  (void)t;
+  out[0] = -p[0] * y[0] + p[1] * y[1] * y[2];
+  out[1] = p[0] * y[0] - p[1] * y[1] * y[2] - p[2] * y[1] * y[1];
+  out[2] = p[2] * y[1] * y[1];
+}
+
+void oracle_cf_robertson_jac(const double* y, double t, const double* p, double* J, double* dT) {
+  (void)t;
+  const double a = p[0], b = p[1], c = p[2];
+  J[0] = -a, J[1] = b * y[2], J[2] = b * y[1];
+  J[3] = a, J[4] = -b * y[2] - 2.0 * c * y[1], J[5] = -b * y[1];
+  J[6] = 0.0, J[7] = 2.0 * c * y[1], J[8] = 0.0;
+  dT[0] = dT[1] = dT[2] = 0.0;
+}
+
+void oracle_cf_robertson_sens_f(const double* y, double t, const double* p, const double* s,
+                                const double* q, double* out) {
+  double J[9], dT[3];
+  oracle_cf_robertson_jac(y, t, p, J, dT);
+  const double y12 = y[1] * y[2], y11 = y[1] * y[1];
+  out[0] = J[0] * s[0] + J[1] * s[1] + J[2] * s[2] - y[0] * q[0] + y12 * q[1];
+  out[1] = J[3] * s[0] + J[4] * s[1] + J[5] * s[2] + y[0] * q[0] - y12 * q[1] - y11 * q[2];
+  out[2] = J[7] * s[1] + y11 * q[2];
+}
+
+void oracle_cf_zero_sens_dt(const double* y, double t, const double* p, const double* s,
+                            const double* q, double* out) {
+  (void)y, (void)t, (void)p, (void)s, (void)q;
+  out[0] = out[1] = out[2] = 0.0;
+}
+
+void oracle_cf_robertson_sens_cpl(const double* y, double t, const double* p, const double* s,
+                                  const double* q, const double* v, double* out) {
+  (void)t;
+  const double b = p[1], c = p[2];
+  const double bs = b * (v[2] * s[1] + v[1] * s[2]);
+  const double dy12 = v[1] * y[2] + y[1] * v[2];
+  out[0] = bs - v[0] * q[0] + dy12 * q[1];
+  out[1] = -bs - 2.0 * c * v[1] * s[1] + v[0] * q[0] - dy12 * q[1] - 2.0 * y[1] * v[1] * q[2];
+  out[2] = 2.0 * c * v[1] * s[1] + 2.0 * y[1] * v[1] * q[2];
+}
+
+/* Van der Pol, y = (x, v), p = (m): f = (v, mu m (1 - x^2) v - x)
+ *   J_y = [[0, 1], [-2 mu m x v - 1, mu m (1 - x^2)]],  J_p = (0, mu (1 - x^2) v)
+ *   D_y(J_y s)[w] = (0, -2 mu m (w_x v + x w_v) s_x - 2 mu m x w_x s_v)
+ *   D_y(J_p q)[w] = (0, mu (-2 x w_x v + (1 - x^2) w_v) q) */
+static const double CF_VDP_MU = 1.0;
+
+void oracle_cf_vdp_f(const double* y, double t, const double* p, double* out) {
+  (void)t;
+  const double x = y[0], v = y[1], m = CF_VDP_MU * p[0];
+  out[0] = v;
+  out[1] = m * (1.0 - x * x) * v - x;
+}
+
+void oracle_cf_vdp_jac(const double* y, double t, const double* p, double* J, double* dT) {
+  (void)t;
+  const double x = y[0], v = y[1], m = CF_VDP_MU * p[0];
+  J[0] = 0.0, J[1] = 1.0;
+  J[2] = -2.0 * m * x * v - 1.0, J[3] = m * (1.0 - x * x);
+  dT[0] = dT[1] = 0.0;
+}
+
+void oracle_cf_vdp_sens_f(const double* y, double t, const double* p, const double* s,
+                          const double* q, double* out) {
+  (void)t;
+  const double x = y[0], v = y[1], m = CF_VDP_MU * p[0];
+  out[0] = s[1];
+  out[1] = (-2.0 * m * x * v - 1.0) * s[0] + m * (1.0 - x * x) * s[1]
+           + CF_VDP_MU * (1.0 - x * x) * v * q[0];
+}
+
+void oracle_cf_vdp_sens_dt(const double* y, double t, const double* p, const double* s,
+                           const double* q, double* out) {
+  (void)y, (void)t, (void)p, (void)s, (void)q;
+  out[0] = out[1] = 0.0;
+}
+
+void oracle_cf_vdp_sens_cpl(const double* y, double t, const double* p, const double* s,
+                            const double* q, const double* w, double* out) {
+  (void)t;
+  const double x = y[0], v = y[1], m = CF_VDP_MU * p[0];
+  out[0] = 0.0;
+  out[1] = -2.0 * m * (w[0] * v + x * w[1]) * s[0] - 2.0 * m * x * w[0] * s[1]
+           + CF_VDP_MU * (-2.0 * x * w[0] * v + (1.0 - x * x) * w[1]) * q[0];
+}
+
 int oracle_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

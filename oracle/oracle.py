@@ -294,6 +294,25 @@ def rodas5p(system, y0, times, params, *, weights=None, first_step=None, rtol=1e
     return hist, dict(accepted_steps=acc, rejected_steps=rej, loop_steps=loop)
 
 
+class ClosedFormCallbacks:
+    """The hand-derived closed-form callbacks in ``oracle.c`` (f, J, the
+    sensitivity rows J_y s + J_p q, their df/dt and the coupling
+    D_y(J_y s + J_p q)[v]) for ``"robertson"`` (3 variables, 3 rates) or
+    ``"vdp"`` (one Van der Pol oscillator, mu = 1.0 * p[0]), in the form
+    ``rodas5p_callbacks`` takes: an independent stand-in for the derivative
+    graphs of ``callbacks.HostCallbacks``."""
+
+    SIZES = {"robertson": (3, 3), "vdp": (2, 1)}
+
+    def __init__(self, name):
+        lib = _lib()
+        self.n, self.n_p = self.SIZES[name]
+        dt = "zero_sens_dt" if name == "robertson" else "vdp_sens_dt"
+        names = (f"{name}_f", f"{name}_jac", f"{name}_sens_f", dt, f"{name}_sens_cpl")
+        self.addresses = tuple(ctypes.cast(getattr(lib, f"oracle_cf_{x}"), ctypes.c_void_p).value
+                               for x in names)
+
+
 def rodas5p_callbacks(cb, y0, times, params, *, direction_columns=(), error_control=True,
                       weights=None, first_step=None, rtol=1e-8, atol=1e-10, max_steps=100000,
                       lu_precision="fp32", pcoeff=0.0, icoeff=1.0, dcoeff=0.0, tf_index=None,

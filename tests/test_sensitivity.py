@@ -177,3 +177,62 @@ def test_stiff_bilinear_sensitivities_against_differences_of_the_plain_solve():
               - O.rodas5p_callbacks(cb, y0, t, pm, **kw)[0][0]) / (2 * h)
         scale = np.abs(fd).max(axis=0) + 1e-300
         assert np.all(np.abs(sens[:, :, k] - fd).max(axis=0) <= 2e-4 * scale)
+
+
+def _callback_values(cb, n, n_p, y, p, s, q, v):
+    """f, J, dT, sens_f, sens_dt and the coupling of a callback set at one point."""
+    import ctypes
+
+    D = ctypes.POINTER(ctypes.c_double)
+
+    def call(addr, n_arrays, *arrays):
+        proto = ctypes.CFUNCTYPE(None, *([D, ctypes.c_double] + [D] * n_arrays))
+        proto(addr)(y.ctypes.data_as(D), 0.3, *[a.ctypes.data_as(D) for a in arrays])
+
+    f, J, dT, sf, sd, cpl = (np.zeros(k) for k in (n, n * n, n, n, n, n))
+    call(cb.addresses[0], 2, p, f)
+    call(cb.addresses[1], 3, p, J, dT)
+    call(cb.addresses[2], 4, p, s, q, sf)
+    call(cb.addresses[3], 4, p, s, q, sd)
+    call(cb.addresses[4], 5, p, s, q, v, cpl)
+    return f, J, dT, sf, sd, cpl
+
+
+@pytest.mark.parametrize("name", ["robertson", "vdp"])
+def test_generated_sensitivity_derivatives_against_hand_derived_closed_forms(name):
+    """The joint solve's derivative code -- J_y s + J_p q, its df/dt and the
+    stage coupling D_y(J_y s + J_p q)[v] -- as the tracer emits it (what the
+    GPU joint kernel is bit-exact with, through the oracle's callback solver)
+    against closed forms derived by hand in oracle.c: equal to rounding at
+    random points, and through the oracle's whole FP64-LU joint integration
+    (every state and parameter direction) the same step counts on every
+    trajectory and histories within 1e-9 relative. A wrong sign or a dropped
+    second-derivative term moves the sensitivities by O(1)."""
+    if name == "robertson":
+        gen = HostCallbacks(S.robertson, 3, 3)
+        y0, p = S.robertson_lognormal_ensemble(300, seed=3)
+        t = S.robertson_benchmark_times()
+        cols, kw = (-1, -1, -1, 0, 1, 2), dict(rtol=1e-6, atol=1e-8)
+    else:
+        gen = HostCallbacks(S.vdp_lattice(1, mu=1.0, d=0.1, wrap=False)[0], 2, 1)
+        rng = np.random.default_rng(1)
+        p = 10.0 ** rng.uniform(0.5, 1.5, size=(200, 1))
+        y0 = rng.uniform(-2.0, 2.0, size=(200, 2))
+        t = np.linspace(0.0, 3.0, 20)
+        cols, kw = (-1, -1, 0), dict(rtol=1e-8, atol=1e-10)
+    cf = O.ClosedFormCallbacks(name)
+    n, n_p = cf.n, cf.n_p
+    rng = np.random.default_rng(7)
+    for _ in range(100):
+        args = (rng.uniform(-2, 2, n), rng.uniform(0.5, 3.0, n_p), rng.normal(size=n),
+                rng.normal(size=n_p), rng.normal(size=n))
+        for a, b in zip(_callback_values(gen, n, n_p, *args), _callback_values(cf, n, n_p, *args)):
+            np.testing.assert_allclose(a, b, rtol=0, atol=1e-14 * max(np.abs(b).max(), 1.0))
+    z0 = np.stack([_joint_y0(r, n, cols) for r in y0])
+    a, sa = O.rodas5p_callbacks(gen, z0, t, p, direction_columns=cols, lu_precision="fp64", **kw)
+    b, sb = O.rodas5p_callbacks(cf, z0, t, p, direction_columns=cols, lu_precision="fp64", **kw)
+    for k in sa:
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+    scale = np.abs(b).max(axis=1, keepdims=True)
+    assert (np.abs(a - b) / scale).max() < 1e-9
+
