@@ -209,8 +209,10 @@ def adjoint_lds_supported(sparsity=None, ordering: str = "amd") -> bool:
     LDS adjoint kernel gave gradients different from the scratch kernel's, for
     a cause not found (the emitted routines are clean under AddressSanitizer on
     the host, the forward kernel with the same LDS layout is exact, and so is
-    the adjoint with either routine alone table driven), so those patterns run
-    the scratch adjoint."""
+    the adjoint with either routine alone table driven; zeroing the LDS stage
+    vectors before every forward step made it exact, so the replay from a
+    checkpoint saw the LDS's earlier contents), so those patterns run the
+    scratch adjoint."""
     if sparsity is None:
         return True
     from . import _sparse_direct
