@@ -113,6 +113,25 @@ def test_uneven_stiff_refined_against_the_oracle(placement, block):
     assert np.abs(hist[:, -1] - h64[:, -1]).max() < 1e-6
 
 
+def test_uneven_stiff_refined_launch_shapes_and_claim_orders_bit_identical():
+    """2,048 spread trajectories of the block arrow: the refined kernel at the
+    size rule's workgroup (32 lanes, stage vectors and right-hand sides in
+    LDS) equals the 64-lane and scratch forms, plain and refill launches, and
+    the refill launch with the pilot claim order."""
+    fn, mask, _, _, y0, p, t = _uneven(2048)
+    kw = dict(rtol=1e-6, atol=1e-8, lu_precision="fp32", lu_refine=True, sparsity=mask,
+              return_stats=True)
+    base, bst = rodas5P.solve(fn, y0, t, p, trajectories_per_block=32, refill=False, **kw)
+    for extra in (dict(trajectories_per_block=32, refill=True),
+                  dict(trajectories_per_block=32, refill=True, work_order="pilot"),
+                  dict(trajectories_per_block=64, refill=True),
+                  dict(placement="rolled", refill=False)):
+        h, st = rodas5P.solve(fn, y0, t, p, **kw, **extra)
+        np.testing.assert_array_equal(h, base, err_msg=str(extra))
+        for k in bst:
+            np.testing.assert_array_equal(st[k], bst[k], err_msg=f"{extra} {k}")
+
+
 def test_refine_is_rejected_where_it_does_not_apply():
     y0, p = S.robertson_lognormal_ensemble(4, seed=1)
     t = S.robertson_benchmark_times()
