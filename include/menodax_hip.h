@@ -54,6 +54,7 @@ void mdx_free(void* p);
 #define MDX_KERNEL_ADJOINT 4      /* a discrete-adjoint kernel, launched by mdx_adjoint_launch */
 #define MDX_KERNEL_HOOK 8         /* runs a save hook: hook_out (N, hook_size) required */
 #define MDX_KERNEL_WAVE 16        /* one trajectory per workgroup of `block` lanes: grid = N */
+#define MDX_KERNEL_RESUMABLE 32   /* reads / writes a step-state buffer (the *_resum* entries) */
 
 /* Register a kernel from a code object: `entry` is its symbol, `block` the
  * workgroup size it was compiled for, `flags` a combination of the
@@ -143,6 +144,34 @@ int32_t mdx_cost_pilot_launch(int64_t handle, void* stream, int64_t n_traj, int3
                               double atol, int32_t steps, const double* weights, double* hist,
                               int32_t* accepted, int32_t* rejected, int32_t* loop_steps,
                               double* cost);
+
+/* mdx_cost_pilot_launch that also leaves each trajectory's step state in
+ * `resume`, (n_vars + 8) x n_traj doubles with the trajectory index fastest:
+ * the state vector, then t, dt, the controller's two previous error norms, the
+ * next save index and the step, accepted and rejected counts. The saves it
+ * passed are already in `hist`. For kernels created with MDX_KERNEL_RESUMABLE
+ * (the one-lane Rodas5P kernel). No counterpart in the reference. */
+int32_t mdx_cost_pilot_launch_resumable(int64_t handle, void* stream, int64_t n_traj,
+                                        int32_t n_vars, int32_t n_params, int32_t n_save,
+                                        int32_t layout, const double* y0, const double* times,
+                                        const double* params, double dt0, double rtol,
+                                        double atol, int32_t steps, const double* weights,
+                                        double* hist, int32_t* accepted, int32_t* rejected,
+                                        int32_t* loop_steps, double* cost, double* resume);
+
+/* mdx_solve_launch_ordered (order may be NULL) whose trajectories start from
+ * the step state mdx_cost_pilot_launch_resumable left in `resume`, into the
+ * same `hist`, so the pilot's steps are not taken again; the results are the
+ * bits of a solve from y0. For kernels created with MDX_KERNEL_RESUMABLE
+ * (one-lane or refill Rodas5P, no save hook). No counterpart in the reference. */
+int32_t mdx_solve_launch_resumed(int64_t handle, void* stream, int32_t grid, int64_t n_traj,
+                                 int32_t n_vars, int32_t n_params, int32_t n_save, int32_t layout,
+                                 const double* y0, const double* times, const double* params,
+                                 double dt0, double rtol, double atol, int32_t max_steps,
+                                 const double* weights, double* hist, int32_t* accepted,
+                                 int32_t* rejected, int32_t* loop_steps, void* workspace,
+                                 int64_t workspace_bytes, const int64_t* order,
+                                 const double* resume);
 
 /* Device memory mdx_adjoint_launch needs as `workspace`: per trajectory,
  * max_ckpt checkpoints of n_vars + 8 doubles and ckpt_every step records of

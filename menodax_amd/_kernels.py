@@ -393,12 +393,16 @@ def _params_in_regs(options, n_params):
     return n_params <= PARAMS_IN_REGS_MAX
 
 
-def _register(code, block, unit_weights, plain_entry, refill_entry, hook=False, wave=False):
+def _register(code, block, unit_weights, plain_entry, refill_entry, hook=False, wave=False,
+              resumable=False):
     """Handles of the one-lane-per-trajectory kernel and of the persistent
-    refill kernel (0 when there is none: a wave-per-trajectory kernel)."""
+    refill kernel (0 when there is none: a wave-per-trajectory kernel);
+    ``resumable`` kernels take a cost pilot's step state (Rodas5P)."""
     flags = _runtime.KERNEL_UNIT_WEIGHTS if unit_weights else 0
     if hook:
         flags |= _runtime.KERNEL_HOOK
+    if resumable and not hook and not wave:
+        flags |= _runtime.KERNEL_RESUMABLE
     plain = _runtime.kernel_create(code, plain_entry, block,
                                    flags | (_runtime.KERNEL_WAVE if wave else 0))
     if refill_entry is None:
@@ -571,7 +575,7 @@ def rodas5p_kernel(ode_fn, n_vars: int, n_params: int, options: KernelOptions) -
     wave = d["placement"]["placement"] == WAVE
     plain, refill = _register(code, d["block"], options.unit_weights, "mdx_rodas5p",
                               None if wave else "mdx_rodas5p_refill",
-                              hook=options.save_hook is not None, wave=wave)
+                              hook=options.save_hook is not None, wave=wave, resumable=True)
     return Built(
         method="rodas5p", plain=plain, refill=refill, block=d["block"], n_vars=n_vars,
         n_params=n_params, unit_weights=options.unit_weights, source=source,

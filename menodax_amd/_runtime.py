@@ -26,6 +26,7 @@ KERNEL_UNIT_WEIGHTS = 2
 KERNEL_ADJOINT = 4
 KERNEL_HOOK = 8
 KERNEL_WAVE = 16
+KERNEL_RESUMABLE = 32
 
 # Every kernel is compiled with contraction off (the fused multiply-adds are
 # explicit in the source) and without packed FP32 instructions.
@@ -99,6 +100,13 @@ def lib():
             vp, vp, vp, vp, vp,
         ]
         L.mdx_cost_pilot_launch.restype = i32
+        L.mdx_cost_pilot_launch_resumable.argtypes = L.mdx_cost_pilot_launch.argtypes + [vp]
+        L.mdx_cost_pilot_launch_resumable.restype = i32
+        L.mdx_solve_launch_resumed.argtypes = [
+            i64, vp, i32, i64, i32, i32, i32, i32, vp, vp, vp, dbl, dbl, dbl, i32, vp,
+            vp, vp, vp, vp, vp, i64, vp, vp,
+        ]
+        L.mdx_solve_launch_resumed.restype = i32
         L.mdx_adjoint_workspace_bytes.argtypes = [i64, i32, i32, i32]
         L.mdx_adjoint_workspace_bytes.restype = i64
         L.mdx_adjoint_launch.argtypes = [
@@ -314,9 +322,20 @@ def adjoint_launch_static(handle, stream, n_traj, n_vars, n_params, n_save, layo
 
 def solve_launch(handle, stream, grid, n_traj, n_vars, n_params, n_save, layout, y0, times, params,
                  dt0, rtol, atol, max_steps, weights, hist, accepted, rejected, loop_steps,
-                 hook_out=0, hook_size=0, workspace=0, workspace_bytes=0, order=0):
+                 hook_out=0, hook_size=0, workspace=0, workspace_bytes=0, order=0, resume=0):
     """Launch through ``mdx_solve_launch`` (``mdx_solve_launch_ordered`` when a
-    claim ``order`` pointer is given); pointers are plain integers."""
+    claim ``order`` pointer is given, ``mdx_solve_launch_resumed`` when a
+    pilot's step-state buffer ``resume`` is); pointers are plain integers."""
+    if resume:
+        if hook_out or hook_size:
+            raise ValueError("a resumed solve runs no save hook")
+        _check(lib().mdx_solve_launch_resumed(
+            handle, stream, int(grid), int(n_traj), int(n_vars), int(n_params), int(n_save),
+            int(layout), y0, times, params, float(dt0), float(rtol), float(atol),
+            int(max_steps), weights or None, hist, accepted, rejected, loop_steps,
+            workspace or None, int(workspace_bytes), order or None, resume),
+            "mdx_solve_launch_resumed")
+        return
     args = (handle, stream, int(grid), int(n_traj), int(n_vars), int(n_params), int(n_save),
             int(layout), y0, times, params, float(dt0), float(rtol), float(atol), int(max_steps),
             weights or None, hist, accepted, rejected, loop_steps, hook_out or None,
@@ -328,13 +347,15 @@ def solve_launch(handle, stream, grid, n_traj, n_vars, n_params, n_save, layout,
 
 
 def cost_pilot_launch(handle, stream, n_traj, n_vars, n_params, n_save, layout, y0, times, params,
-                      dt0, rtol, atol, steps, weights, hist, accepted, rejected, loop_steps, cost):
-    """Launch through ``mdx_cost_pilot_launch``; pointers are plain integers."""
-    _check(
-        lib().mdx_cost_pilot_launch(
-            handle, stream, int(n_traj), int(n_vars), int(n_params), int(n_save), int(layout),
+                      dt0, rtol, atol, steps, weights, hist, accepted, rejected, loop_steps, cost,
+                      resume=0):
+    """Launch through ``mdx_cost_pilot_launch`` (``_resumable`` when a
+    step-state buffer ``resume`` is given); pointers are plain integers."""
+    args = (handle, stream, int(n_traj), int(n_vars), int(n_params), int(n_save), int(layout),
             y0, times, params, float(dt0), float(rtol), float(atol), int(steps),
-            weights or None, hist, accepted, rejected, loop_steps, cost,
-        ),
-        "mdx_cost_pilot_launch",
-    )
+            weights or None, hist, accepted, rejected, loop_steps, cost)
+    if resume:
+        _check(lib().mdx_cost_pilot_launch_resumable(*args, resume),
+               "mdx_cost_pilot_launch_resumable")
+    else:
+        _check(lib().mdx_cost_pilot_launch(*args), "mdx_cost_pilot_launch")

@@ -51,6 +51,7 @@ struct SolveArgs {
   unsigned long long* queue;
   const int64_t* order;
   double* cost;
+  double* resume;
   double dt0, rtol, atol;
   int64_t n_traj;
   int64_t y0_ts, y0_vs;
@@ -61,7 +62,7 @@ struct SolveArgs {
   int32_t max_steps;
   int32_t n_save;
 };
-static_assert(sizeof(SolveArgs) == 12 * 8 + 3 * 8 + 8 + 11 * 8 + 8, "SolveArgs layout");
+static_assert(sizeof(SolveArgs) == 13 * 8 + 3 * 8 + 8 + 11 * 8 + 8, "SolveArgs layout");
 
 // Mirrors mdx::AdjointArgs in kernels/mdx_common.hpp (both adjoint kernels).
 struct AdjointArgs {
@@ -105,6 +106,7 @@ struct Kernel {
   bool adjoint = false;
   bool hook = false;
   bool wave = false;  // one trajectory per workgroup
+  bool resumable = false;  // reads and writes a step-state buffer (SolveArgs::resume)
   std::map<int, Loaded> per_device;
 };
 
@@ -246,6 +248,7 @@ int32_t mdx_kernel_create(const void* code, size_t code_size, const char* entry,
   k->adjoint = (flags & MDX_KERNEL_ADJOINT) != 0;
   k->hook = (flags & MDX_KERNEL_HOOK) != 0;
   k->wave = (flags & MDX_KERNEL_WAVE) != 0;
+  k->resumable = (flags & MDX_KERNEL_RESUMABLE) != 0;
   if (k->adjoint && k->persistent)
     return fail(1, "mdx_kernel_create: an adjoint kernel is not a persistent kernel");
   if (k->wave && (k->persistent || k->adjoint))
@@ -328,7 +331,7 @@ int32_t launch_impl(int64_t handle, void* stream, int32_t grid, int64_t n_traj, 
                     double atol, int32_t max_steps, const double* weights, double* hist,
                     int32_t* accepted, int32_t* rejected, int32_t* loop_steps, double* hook_out,
                     int32_t hook_size, void* workspace, int64_t workspace_bytes,
-                    const int64_t* order, double* cost);
+                    const int64_t* order, double* cost, double* resume);
 }  // namespace
 
 int32_t mdx_solve_launch_ordered(int64_t handle, void* stream, int32_t grid, int64_t n_traj,
@@ -341,7 +344,23 @@ int32_t mdx_solve_launch_ordered(int64_t handle, void* stream, int32_t grid, int
                                  const int64_t* order) {
   return launch_impl(handle, stream, grid, n_traj, n_vars, n_params, n_save, layout, y0, times,
                      params, dt0, rtol, atol, max_steps, weights, hist, accepted, rejected,
-                     loop_steps, hook_out, hook_size, workspace, workspace_bytes, order, nullptr);
+                     loop_steps, hook_out, hook_size, workspace, workspace_bytes, order, nullptr,
+                     nullptr);
+}
+
+int32_t mdx_solve_launch_resumed(int64_t handle, void* stream, int32_t grid, int64_t n_traj,
+                                 int32_t n_vars, int32_t n_params, int32_t n_save, int32_t layout,
+                                 const double* y0, const double* times, const double* params,
+                                 double dt0, double rtol, double atol, int32_t max_steps,
+                                 const double* weights, double* hist, int32_t* accepted,
+                                 int32_t* rejected, int32_t* loop_steps, void* workspace,
+                                 int64_t workspace_bytes, const int64_t* order,
+                                 const double* resume) {
+  if (!resume) return fail(1, "mdx_solve_launch_resumed: null step-state buffer");
+  return launch_impl(handle, stream, grid, n_traj, n_vars, n_params, n_save, layout, y0, times,
+                     params, dt0, rtol, atol, max_steps, weights, hist, accepted, rejected,
+                     loop_steps, nullptr, 0, workspace, workspace_bytes, order, nullptr,
+                     const_cast<double*>(resume));
 }
 
 int32_t mdx_cost_pilot_launch(int64_t handle, void* stream, int64_t n_traj, int32_t n_vars,
@@ -354,7 +373,22 @@ int32_t mdx_cost_pilot_launch(int64_t handle, void* stream, int64_t n_traj, int3
   if (steps < 1) return fail(1, "mdx_cost_pilot_launch: steps must be positive");
   return launch_impl(handle, stream, 0, n_traj, n_vars, n_params, n_save, layout, y0, times,
                      params, dt0, rtol, atol, steps, weights, hist, accepted, rejected,
-                     loop_steps, nullptr, 0, nullptr, 0, nullptr, cost);
+                     loop_steps, nullptr, 0, nullptr, 0, nullptr, cost, nullptr);
+}
+
+int32_t mdx_cost_pilot_launch_resumable(int64_t handle, void* stream, int64_t n_traj,
+                                        int32_t n_vars, int32_t n_params, int32_t n_save,
+                                        int32_t layout, const double* y0, const double* times,
+                                        const double* params, double dt0, double rtol,
+                                        double atol, int32_t steps, const double* weights,
+                                        double* hist, int32_t* accepted, int32_t* rejected,
+                                        int32_t* loop_steps, double* cost, double* resume) {
+  if (!cost) return fail(1, "mdx_cost_pilot_launch_resumable: null cost buffer");
+  if (!resume) return fail(1, "mdx_cost_pilot_launch_resumable: null step-state buffer");
+  if (steps < 1) return fail(1, "mdx_cost_pilot_launch_resumable: steps must be positive");
+  return launch_impl(handle, stream, 0, n_traj, n_vars, n_params, n_save, layout, y0, times,
+                     params, dt0, rtol, atol, steps, weights, hist, accepted, rejected,
+                     loop_steps, nullptr, 0, nullptr, 0, nullptr, cost, resume);
 }
 
 namespace {
@@ -364,7 +398,7 @@ int32_t launch_impl(int64_t handle, void* stream, int32_t grid, int64_t n_traj, 
                     double atol, int32_t max_steps, const double* weights, double* hist,
                     int32_t* accepted, int32_t* rejected, int32_t* loop_steps, double* hook_out,
                     int32_t hook_size, void* workspace, int64_t workspace_bytes,
-                    const int64_t* order, double* cost) {
+                    const int64_t* order, double* cost, double* resume) {
   Kernel* k = nullptr;
   if (int32_t rc = lookup(handle, &k)) return rc;
   if (n_traj < 0 || n_vars < 1 || n_params < 0 || n_save < 1)
@@ -391,6 +425,9 @@ int32_t launch_impl(int64_t handle, void* stream, int32_t grid, int64_t n_traj, 
   if (cost && (k->persistent || k->hook))
     return fail(1, "mdx_cost_pilot_launch: a cost pilot runs a one-lane-per-trajectory kernel "
                    "without a save hook");
+  if (resume && !k->resumable)
+    return fail(1, "mdx_solve_launch: a step-state buffer is for kernels created with "
+                   "MDX_KERNEL_RESUMABLE (the one-lane and refill Rodas5P kernels)");
   if (k->persistent && (!workspace || workspace_bytes < 8))
     return fail(1, "mdx_solve_launch: a persistent kernel needs a workspace of at least 8 bytes");
   if (n_traj == 0) return 0;
@@ -409,6 +446,7 @@ int32_t launch_impl(int64_t handle, void* stream, int32_t grid, int64_t n_traj, 
   a.queue = (unsigned long long*)workspace;
   a.order = order;
   a.cost = cost;
+  a.resume = resume;
   a.dt0 = dt0;
   a.rtol = rtol;
   a.atol = atol;

@@ -1093,6 +1093,25 @@ __device__ __forceinline__ void init(State& s, const SolveArgs& a, i64 i, const 
   s.n_steps = 0;
   s.accepted = 0;
   s.rejected = 0;
+#if !MDX_ADJOINT
+  if (a.resume && !a.cost) {
+    // where a cost pilot left the trajectory: everything a step carries to the
+    // next, so the solve continues with the bits an unbroken one has (the
+    // pilot wrote the saves it passed)
+    const double* __restrict__ r = a.resume + i;
+    const i64 st = a.n_traj;
+#pragma unroll
+    for (int j = 0; j < NA; ++j) YJ(s, j) = r[j * st];
+    s.t = r[(NA + 0) * st];
+    s.dt = r[(NA + 1) * st];
+    s.err_prev = r[(NA + 2) * st];
+    s.err_prev2 = r[(NA + 3) * st];
+    s.save_idx = (int)r[(NA + 4) * st];
+    s.n_steps = (int)r[(NA + 5) * st];
+    s.accepted = (int)r[(NA + 6) * st];
+    s.rejected = (int)r[(NA + 7) * st];
+  }
+#endif
 }
 
 __device__ __forceinline__ bool active(const State& s, const SolveArgs& a) {
@@ -1493,6 +1512,20 @@ __device__ __forceinline__ void finish(State& s, const PA& p, const SolveArgs& a
     // a cost pilot (max_steps is its step budget): the steps left at the
     // current step size instead of the saves and counters, 0 once finished
     a.cost[i] = (s.save_idx < a.n_save && s.t < s.tf) ? (s.tf - s.t) / s.dt : 0.0;
+    if (a.resume) {
+      double* __restrict__ r = a.resume + i;
+      const i64 st = a.n_traj;
+#pragma unroll
+      for (int j = 0; j < NA; ++j) r[j * st] = YJ(s, j);
+      r[(NA + 0) * st] = s.t;
+      r[(NA + 1) * st] = s.dt;
+      r[(NA + 2) * st] = s.err_prev;
+      r[(NA + 3) * st] = s.err_prev2;
+      r[(NA + 4) * st] = (double)s.save_idx;
+      r[(NA + 5) * st] = (double)s.n_steps;
+      r[(NA + 6) * st] = (double)s.accepted;
+      r[(NA + 7) * st] = (double)s.rejected;
+    }
     return;
   }
   while (s.save_idx < a.n_save) {

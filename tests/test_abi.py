@@ -97,6 +97,26 @@ def test_a_cost_pilot_runs_a_one_lane_kernel(handles):
         _runtime.cost_pilot_launch(handles[0], None, **dict(kw, cost=0))
 
 
+def test_a_step_state_buffer_needs_a_resumable_kernel(handles):
+    """The pilot's step state goes only to kernels created with
+    MDX_KERNEL_RESUMABLE (the fixture's are not), and both resumable entry
+    points refuse a null buffer; all before touching the GPU."""
+    kw = dict(n_traj=4, n_vars=3, n_params=3, n_save=5, layout=1, y0=8, times=8, params=8,
+              dt0=0.0, rtol=1e-6, atol=1e-8, steps=10, weights=8, hist=8, accepted=8,
+              rejected=8, loop_steps=8, cost=8)
+    with pytest.raises(_runtime.NativeError, match="MDX_KERNEL_RESUMABLE"):
+        _runtime.cost_pilot_launch(handles[0], None, **kw, resume=8)
+    with pytest.raises(_runtime.NativeError, match="MDX_KERNEL_RESUMABLE"):
+        _launch(handles[1], workspace=8, workspace_bytes=256, resume=8)
+    lib = _runtime.lib()
+    args = [handles[0], None, 4, 3, 3, 5, 1, 8, 8, 8, 0.0, 1e-6, 1e-8, 10, 8, 8, 8, 8, 8, 8]
+    assert lib.mdx_cost_pilot_launch_resumable(*args, None) != 0
+    assert b"null step-state" in lib.mdx_last_error()
+    assert lib.mdx_solve_launch_resumed(handles[1], None, 0, 4, 3, 3, 5, 1, 8, 8, 8, 0.0, 1e-6,
+                                        1e-8, 10, 8, 8, 8, 8, 8, 8, 256, None, None) != 0
+    assert b"null step-state" in lib.mdx_last_error()
+
+
 def test_work_order_must_be_a_permutation():
     import torch
 
