@@ -1100,7 +1100,14 @@ __device__ __forceinline__ void init(State& s, const SolveArgs& a, i64 i, const 
     // pilot wrote the saves it passed)
     const double* __restrict__ r = a.resume + i;
     const i64 st = a.n_traj;
+    // a loop in the rolled kernel (its state lies in LDS or scratch, and an
+    // unrolled copy would hold every element in flight beside the step's
+    // registers); unrolled where the state is registers
+#if MDX_ROLLED
+#pragma unroll 1
+#else
 #pragma unroll
+#endif
     for (int j = 0; j < NA; ++j) YJ(s, j) = r[j * st];
     s.t = r[(NA + 0) * st];
     s.dt = r[(NA + 1) * st];
@@ -1515,7 +1522,11 @@ __device__ __forceinline__ void finish(State& s, const PA& p, const SolveArgs& a
     if (a.resume) {
       double* __restrict__ r = a.resume + i;
       const i64 st = a.n_traj;
+#if MDX_ROLLED
+#pragma unroll 1
+#else
 #pragma unroll
+#endif
       for (int j = 0; j < NA; ++j) r[j * st] = YJ(s, j);
       r[(NA + 0) * st] = s.t;
       r[(NA + 1) * st] = s.dt;
