@@ -116,6 +116,36 @@ def test_tf_index_ends_each_trajectory_at_its_own_time():
     assert (out[1, stop:, 0] == out[1, stop - 1, 0]).all()
 
 
+@pytest.mark.parametrize("lu", ["fp32", "fp64"])
+def test_a_resumed_pilot_gives_the_unordered_solves_bits(lu):
+    """``work_order="pilot"`` continues every Rodas5P trajectory from the step
+    state its pilot left (``mdx_solve_launch_resumed``). Against the unordered
+    refill solve, bit for bit with the same counters: trajectories that end
+    inside the pilot (their own end time from the parameters, tf_index), ones
+    that end after it, and a step budget (max_steps) reached inside the
+    pilot's length and after it."""
+    rng = np.random.default_rng(11)
+    t = np.linspace(0.0, 1.0, 11)
+    n = 300
+    params = np.column_stack([10.0 * rng.uniform(0.5, 2.0, n), 5.0 * rng.uniform(0.5, 2.0, n),
+                              rng.uniform(0.02, 1.0, n)])
+    kw = dict(tf_index=2, rtol=1e-10, atol=1e-12, lu_precision=lu, return_stats=True)
+    base, bst = rodas5P.solve(S.forced_decay, np.zeros((n, 1)), t, params, refill=True, **kw)
+    assert (bst["accepted_steps"] < rodas5P.PILOT_STEPS).any()
+    assert (bst["accepted_steps"] > rodas5P.PILOT_STEPS).any()
+    hist, st = rodas5P.solve(S.forced_decay, np.zeros((n, 1)), t, params, work_order="pilot",
+                             **kw)
+    _same(hist, st, base, bst)
+    y0, p = S.robertson_lognormal_ensemble(300, seed=6)
+    tr = S.robertson_benchmark_times()
+    for max_steps in (rodas5P.PILOT_STEPS // 2, 3 * rodas5P.PILOT_STEPS):
+        kw = dict(rtol=1e-6, atol=1e-8, lu_precision=lu, max_steps=max_steps, return_stats=True)
+        base, bst = rodas5P.solve(S.robertson, y0, tr, p, refill=True, **kw)
+        hist, st = rodas5P.solve(S.robertson, y0, tr, p, work_order="pilot", **kw)
+        _same(hist, st, base, bst)
+        assert (st["loop_steps"] <= max_steps).all()
+
+
 def test_max_steps_exhaustion_holds_the_last_state():
     y0, p = S.robertson_lognormal_ensemble(100, seed=4)
     t = S.robertson_benchmark_times()
