@@ -332,6 +332,52 @@ def test_every_placement_is_bit_exact(placement, lu):
         _same(hist, st, ref, rst)
 
 
+@pytest.mark.parametrize("block", [31, 48, 100])
+@pytest.mark.parametrize("placement", ["registers", "lds"])
+def test_workgroups_of_any_size_are_bit_exact(placement, block):
+    """trajectories_per_block takes any positive count (modax/rodas5P.py:120-134).
+    A workgroup filling part of a wavefront (31, 48 lanes) or one and a part
+    (100) changes which lanes share a wave and an LDS block -- the [element][lane]
+    rows then have that many lanes -- and never a trajectory's arithmetic:
+    registers and the rolled LDS step, both launch shapes and a claim order,
+    against the oracle; Tsit5's LDS step too."""
+    from menodax_amd import _kernels, rodas5P as R5
+
+    fn, mask, (rows, cols, vals, pidx) = S.arrow_system(24)
+    n_p = S.arrow_n_params(24)
+    y0, lam = S.arrow_ensemble(400, n=24)
+    t = np.linspace(0.0, 10.0, 7)
+    system = O.linquad(24, rows, cols, vals, 0.01, off_pidx=pidx, n_p=n_p)
+    layout = _structure(mask, 24, "amd")
+    ref, rst = O.rodas5p(system, y0, t, lam, layout=layout, rtol=1e-6, atol=1e-8,
+                         lu_precision="fp32")
+    kw = dict(sparsity=mask, rtol=1e-6, atol=1e-8, lu_precision="fp32", return_stats=True,
+              placement=placement, trajectories_per_block=block)
+    for extra in (dict(refill=False), dict(refill=True), dict(work_order="pilot")):
+        hist, st = rodas5P.solve(fn, y0, t, lam, **kw, **extra)
+        _same(hist, st, ref, rst)
+    if placement != "lds":
+        return
+    import torch
+
+    fv, _ = S.vdp_lattice(12, mu=1.0)
+    rng = np.random.default_rng(9)
+    y0v = rng.uniform(-2, 2, size=(300, 24))
+    pv = rng.uniform(0.5, 3.0, size=(300, 1))
+    tv = np.linspace(0.0, 2.0, 9)
+    ref, rst = O.tsit5(O.vdp_lattice(12, mu=1.0), y0v, tv, pv, rtol=1e-7, atol=1e-9)
+    built = _kernels.tsit5_kernel(fv, 24, 1, _kernels.Tsit5Options(
+        placement="lds", trajectories_per_block=block))
+    assert built.block == block
+    dev = torch.device("cuda")
+    for refill in (False, True):
+        out = R5.launch(built, torch.tensor(y0v, device=dev), torch.tensor(tv, device=dev),
+                        torch.tensor(pv, device=dev), 300, dt0=0.0, rtol=1e-7, atol=1e-9,
+                        max_steps=100000, refill=refill)
+        hist, st = R5.finish(out, True, True)
+        _same(hist, st, ref, rst)
+
+
 @pytest.mark.parametrize("lu", ["fp32", "fp64"])
 def test_the_tableau_in_registers_changes_no_bit(lu):
     """The unrolled step with its coefficients copied into registers once per
