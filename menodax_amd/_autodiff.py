@@ -166,11 +166,58 @@ class AdjointJob:
     ckpt_every: int = DEFAULT_CKPT_EVERY
 
 
-def adjoint_gradients(job: AdjointJob, y0_t, p_t, gbar, accepted):
+#: the most device memory a checkpoint store sized without the solve's step
+#: counts may take: 8 GiB, and at most a quarter of what is free
+CHECKPOINT_STORE_BUDGET = 8 << 30
+
+
+def _checkpoint_store_guess(n, n_vars, ckpt_every, max_steps) -> int:
+    """Checkpoints to provide when the solve's accepted-step counts are not
+    known: enough for ``max_steps`` accepted steps (then no trajectory can
+    outrun the store), or what ``CHECKPOINT_STORE_BUDGET`` holds."""
+    import torch
+
+    from . import _runtime
+
+    bound = max(int(max_steps), 0) // ckpt_every + 1
+    if n == 0:
+        return 1
+    one = _runtime.adjoint_workspace_bytes(n, n_vars, ckpt_every, 1)
+    per = _runtime.adjoint_workspace_bytes(n, n_vars, ckpt_every, 2) - one
+    free, _ = torch.cuda.mem_get_info()
+    budget = min(CHECKPOINT_STORE_BUDGET, free // 4)
+    return max(1, min(bound, 1 + (budget - one) // per))
+
+
+def adjoint_gradients(job: AdjointJob, y0_t, p_t, gbar, accepted=None):
     """dL/dy0 ``(N, n_vars)`` and dL/dparams ``(N, n_params)`` for the cotangent
-    ``gbar`` ``(N, n_save, n_vars)`` of the solve's history; ``accepted`` are
-    that solve's per-trajectory accepted-step counts (they size the
-    checkpoint store)."""
+    ``gbar`` ``(N, n_save, n_vars)`` of the solve's history. ``accepted`` are
+    that solve's per-trajectory accepted-step counts, which size the
+    checkpoint store; without them the store is sized by
+    ``_checkpoint_store_guess`` and the adjoint kernel's own forward pass is
+    the only one, unless a trajectory outruns that store: then a forward
+    solve counts the steps and the adjoint runs again (the gradients do not
+    depend on the store's size)."""
+    c = int(job.ckpt_every)
+    if c < 1:
+        raise ValueError("ckpt_every must be positive")
+    if accepted is not None:
+        return _adjoint_run(job, y0_t, p_t, gbar, int(accepted.max().item()) // c + 1 if job.n
+                            else 1, must_fit=True)
+    guess = _checkpoint_store_guess(job.n, job.n_vars, c, job.launch_kw["max_steps"])
+    out = _adjoint_run(job, y0_t, p_t, gbar, guess, must_fit=False)
+    if out is not None:
+        return out
+    from .rodas5P import launch
+
+    fwd = launch(job.built, y0_t, job.times, p_t, job.n, weights=job.weights, **job.launch_kw)
+    return _adjoint_run(job, y0_t, p_t, gbar, int(fwd.accepted.max().item()) // c + 1,
+                        must_fit=True)
+
+
+def _adjoint_run(job: AdjointJob, y0_t, p_t, gbar, max_ckpt, must_fit):
+    """One adjoint launch with room for ``max_ckpt`` checkpoints per
+    trajectory; None when a trajectory outran them and ``must_fit`` is false."""
     import torch
 
     from . import _runtime
@@ -178,9 +225,6 @@ def adjoint_gradients(job: AdjointJob, y0_t, p_t, gbar, accepted):
     dev = y0_t.device
     n, nv, np_ = job.n, job.n_vars, job.n_params
     c = int(job.ckpt_every)
-    if c < 1:
-        raise ValueError("ckpt_every must be positive")
-    max_ckpt = int(accepted.max().item()) // c + 1 if n else 1
     ws_bytes = _runtime.adjoint_workspace_bytes(n, nv, c, max_ckpt)
     ws = torch.empty(max(ws_bytes, 8), dtype=torch.uint8, device=dev)
     y0_s = _common.soa(y0_t)
@@ -199,6 +243,8 @@ def adjoint_gradients(job: AdjointJob, y0_t, p_t, gbar, accepted):
         gy.data_ptr(), gp.data_ptr(), status.data_ptr(), c, max_ckpt, ws.data_ptr(), ws_bytes)
     bad = int((status != 0).sum().item())
     if bad:
+        if not must_fit:
+            return None
         raise RuntimeError(f"adjoint: {bad} trajectories outran their checkpoint store; "
                            "their forward pass took more accepted steps than the primal solve")
     return gy.t(), gp[:np_].t()

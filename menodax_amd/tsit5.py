@@ -179,12 +179,11 @@ def adjoint_gradients(ode_fn, y0, t_span, params, hist_cotangent, *, ckpt_every=
         times=times, n=n, n_vars=n_vars, n_params=n_params, weights=weights,
         launch_kw=launch_kw, ckpt_every=int(ckpt_every))
     y0_d, p_d = y0_t.detach(), p_t.detach()
-    out = launch(job.built, y0_d, times, p_d, n, weights=weights, **launch_kw)
     g = _common.as_f64_device(hist_cotangent, dev)
     if tuple(g.shape) != (n, times.numel(), n_vars):
         raise ValueError(f"hist_cotangent must be (N, n_save, n_vars) = "
                          f"{(n, times.numel(), n_vars)}; got {tuple(g.shape)}")
-    gy, gp = _autodiff.adjoint_gradients(job, y0_d, p_d, g, out.accepted)
+    gy, gp = _autodiff.adjoint_gradients(job, y0_d, p_d, g)
     if _common.wants_numpy(y0, params, t_span, hist_cotangent):
         return gy.cpu().numpy(), gp.cpu().numpy()
     return gy, gp

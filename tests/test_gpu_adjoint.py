@@ -92,6 +92,45 @@ def test_gradients_do_not_depend_on_the_checkpoint_interval():
         np.testing.assert_array_equal(got[1], ref[1])
 
 
+def test_a_checkpoint_store_too_small_for_a_trajectory_is_sized_again(monkeypatch):
+    """adjoint_gradients sizes the checkpoint store from max_steps within a
+    memory budget, so its adjoint kernel's forward pass is the only one. With
+    the budget cut to one checkpoint the trajectories outrun it: a forward
+    solve then counts their steps and the adjoint runs again, with the same
+    bits as the store sized at once."""
+    fn, y0, mu, t, target, kw = _vdp_problem(1, 32, seed=6)
+    hist = rodas5P.solve(fn, y0, t, mu, **kw)
+    _, g = _loss_and_cotangent(hist, target)
+    store_resized_bits(monkeypatch, lambda: rodas5P.adjoint_gradients(fn, y0, t, mu, g,
+                                                                      ckpt_every=16, **kw))
+
+
+def store_resized_bits(monkeypatch, run):
+    """``run()`` (an adjoint_gradients call with ckpt_every=16 and the default
+    max_steps) as sized from max_steps, then with a one-checkpoint budget:
+    one launch, then a failed launch and a resized one, the same bits."""
+    from menodax_amd import _autodiff
+
+    runs = []
+    real = _autodiff._adjoint_run
+
+    def spy(job, y0_t, p_t, gbar, max_ckpt, must_fit):
+        out = real(job, y0_t, p_t, gbar, max_ckpt, must_fit)
+        runs.append((max_ckpt, out is not None))
+        return out
+
+    monkeypatch.setattr(_autodiff, "_adjoint_run", spy)
+    ref = run()
+    assert runs == [(100000 // 16 + 1, True)]  # room for max_steps accepted steps
+    monkeypatch.setattr(_autodiff, "CHECKPOINT_STORE_BUDGET", 1)
+    runs.clear()
+    got = run()
+    assert [r[1] for r in runs] == [False, True] and runs[0][0] == 1 and runs[1][0] > 1
+    assert np.all(np.isfinite(ref[0])) and np.all(np.isfinite(ref[1]))
+    np.testing.assert_array_equal(got[0], ref[0])
+    np.testing.assert_array_equal(got[1], ref[1])
+
+
 @pytest.mark.parametrize("lu", ["fp64", "fp32"])
 def test_adjoint_placements_give_the_same_bits(lu):
     # the unrolled register kernel (chosen for 2 variables) against the rolled

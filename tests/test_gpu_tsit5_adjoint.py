@@ -34,6 +34,18 @@ def _loss_and_cotangent(hist, target):
     return float((r * r).sum()), 2.0 * r
 
 
+def test_a_checkpoint_store_too_small_for_a_trajectory_is_sized_again(monkeypatch):
+    """The store sized from max_steps, and re-sized by a forward solve's step
+    counts when a one-checkpoint budget is outrun: the same bits."""
+    from tests.test_gpu_adjoint import store_resized_bits
+
+    y0, rho, t, target, kw = _lorenz_problem(32, seed=6)
+    hist = tsit5.solve(S.lorenz, y0, t, rho, **kw)
+    _, g = _loss_and_cotangent(hist, target)
+    store_resized_bits(monkeypatch, lambda: tsit5.adjoint_gradients(S.lorenz, y0, t, rho, g,
+                                                                    ckpt_every=16, **kw))
+
+
 def test_adjoint_matches_forward_sensitivities():
     y0, rho, t, target, kw = _lorenz_problem(64)
     hist, sens = tsit5.solve_sensitivities(S.lorenz, y0, t, rho, **kw)
