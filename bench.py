@@ -198,7 +198,7 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, err_sample=4096, tight_sample=
     def timed(key, built, refill):
         outs = rodas5P.launch(built, y0_t, t_t, p_t, n, dt0=0.0, rtol=RTOL, atol=ATOL,
                               max_steps=100000, refill=refill)
-        ws = outs._keep[-1]
+        ws = outs.workspace
 
         def run():
             return rodas5P.launch(built, y0_t, t_t, p_t, n, dt0=0.0, rtol=RTOL, atol=ATOL,
@@ -1027,7 +1027,7 @@ def main():
         use_refill = rodas5P._use_refill(built, args.n) if refill is None else refill
         outputs = rodas5P.launch(built, y0_v, times, p_v, args.n, dt0=0.0, rtol=RTOL, atol=ATOL,
                                  max_steps=100000, refill=use_refill)
-        ws = outputs._keep[-1]
+        ws = outputs.workspace
 
         def run(built=built, outputs=outputs, ws=ws, use_refill=use_refill):
             return rodas5P.launch(built, y0_v, times, p_v, args.n, dt0=0.0, rtol=RTOL, atol=ATOL,

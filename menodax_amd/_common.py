@@ -155,6 +155,9 @@ class Outputs:
     rejected: object
     loop: object
     hook_out: object = None  # (hook_size, N) storage when a save hook runs
+    #: the launch's workspace (a persistent kernel's refill counter), for a
+    #: caller that launches again into the same outputs
+    workspace: object = None
 
     def ensemble_view(self):
         """``(N, n_save, n_vars)`` view of the structure-of-arrays history."""

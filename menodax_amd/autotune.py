@@ -131,7 +131,7 @@ def tune(ode_fn, y0, t_span, params, *, sample=None, reps=2, rtol=1e-8, atol=1e-
                 raise RuntimeError(f"placement {placement}, block {block}, tableau in registers "
                                    f"{tab}, refill {refill} changed the result; placements "
                                    "must be bit-identical")
-            ws = out._keep[-1]
+            ws = out.workspace
             t0 = time.perf_counter()
             for _ in range(reps):
                 launch(built, y0_t, times, p_t, n, refill=refill, grid=grid, outputs=out,

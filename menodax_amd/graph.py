@@ -69,7 +69,7 @@ class CapturedSolve:
             self._kw["order"] = _common.work_order(work_order, n, dev).clone()
         # eager launch: loads the module, allocates outputs and workspace
         self._out = launch(self.built, self._y0.t(), self._times, self._p.t(), n, **self._kw)
-        self._ws = self._out._keep[-1]
+        self._ws = self._out.workspace
         self._graph = None
         self._as_numpy = _common.wants_numpy(y0, params, t_span)
 

@@ -23,6 +23,8 @@ def test_replays_match_the_eager_solve(refill):
     kw = dict(rtol=1e-6, atol=1e-8, lu_precision="fp32", refill=refill)
     eager, est = rodas5P.solve(S.robertson, y0, t, p, return_stats=True, **kw)
     cs = CapturedSolve(S.robertson, y0, t, p, **kw)
+    # the persistent launch's counter: the eager launch's, which the graph reuses
+    assert (cs._ws is not None) == refill
     for k in range(3):
         hist, st = cs.run(return_stats=True)
         np.testing.assert_array_equal(hist, eager)

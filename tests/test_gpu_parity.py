@@ -523,6 +523,33 @@ def test_a_claim_order_changes_no_bit():
         _same(hist, st, ref, rst)
 
 
+def test_a_large_ensemble_claims_in_the_resumed_pilot_order_by_default(monkeypatch):
+    """Given no work_order, a solve whose ensemble size picks the persistent
+    launch (20,000 block-arrow trajectories of 50 variables against 16,384
+    resident lanes) ranks its trajectories with the resumed pilot, and gives
+    the bits of the same solve claimed in index order; a small ensemble, or an
+    explicit refill=True, runs no pilot."""
+    fn, mask, _ = S.arrow_system(50)
+    y0, lam = S.arrow_ensemble(20000)
+    t = np.linspace(0.0, 10.0, 20)
+    kw = dict(sparsity=mask, rtol=1e-6, atol=1e-8, lu_precision="fp32", return_stats=True)
+    pilots = []
+    real = rodas5P._pilot_order
+
+    def spy(*a, **k):
+        pilots.append(a[4])  # the ensemble size
+        return real(*a, **k)
+
+    monkeypatch.setattr(rodas5P, "_pilot_order", spy)
+    hist, st = rodas5P.solve(fn, y0, t, lam, **kw)
+    assert pilots == [20000]
+    ref, rst = rodas5P.solve(fn, y0, t, lam, refill=True, **kw)
+    assert pilots == [20000]
+    _same(hist, st, ref, {k: np.asarray(v) for k, v in rst.items()})
+    rodas5P.solve(fn, y0[:1000], t, lam[:1000], **kw)
+    assert pilots == [20000]
+
+
 @pytest.mark.parametrize("ordered", [False, True])
 @pytest.mark.parametrize("grid", [1, 7, 300])
 def test_refill_claims_every_trajectory_exactly_once(grid, ordered):
