@@ -33,7 +33,7 @@ The JSON line also carries
     (checkpoint every 64 accepted steps) and by the forward-sensitivity joint
     solve, with their agreement and a central-difference check on a sample.
 
-``python bench.py --gpus N`` with N > 1 exits: this build runs on one GPU.
+``python bench.py --gpus N`` with N > 1 exits: the benchmark runs on one GPU.
 """
 
 from __future__ import annotations
@@ -116,7 +116,8 @@ def time_solves(run, steps, warmup, torch):
     return out, wall, float(np.mean(event_ms))
 
 
-#: the north-star bound on the endpoint error against the FP64 oracle
+#: the bound the uneven stiff solves are held to: endpoint error against the
+#: FP64 oracle
 ERROR_TARGET = 1e-6
 
 
@@ -374,7 +375,7 @@ def uneven_stiff(torch, dev, O, log_fn, n=100000, err_sample=4096, tight_sample=
         **res,
         "fp32_refill_over_fp64_plain": fp64_ms / res["fp32_refill"]["ms"],
         "fp32_refill_piloted_over_fp64_plain": fp64_ms / res["fp32_refill_piloted"]["ms"],
-        "north_star_target": target,
+        "speed_and_error_target": target,
         "refill_bit_identical_to_plain": same,
         "attempted_steps": {"p1": float(np.percentile(steps, 1)),
                             "median": float(np.median(steps)),
@@ -828,7 +829,7 @@ REF_DIVERGENCE_MS = {
 def robertson_divergence(torch, log_fn, reps=5):
     """The reference's stiff Robertson divergence benchmark (robertson
     make_scenario at divergence 1.0 and 3.0, seed 42, 30,000 trajectories,
-    first_step 1e-4, rtol 1e-6, atol 1e-8, FP32 LU) through this build's launch
+    first_step 1e-4, rtol 1e-6, atol 1e-8, FP32 LU) through the engine's launch
     shapes: the one-lane-per-trajectory kernel on the ensemble as given, on the
     ensemble pre-sorted by attempted steps (the reference's sorted row: the
     inputs reordered, most steps first), the persistent refill kernel, the
@@ -948,7 +949,7 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(
             "bench.py measures one GPU: trajectories are independent, so a multi-GPU run "
-            "is N/G trajectories per GPU with no exchange (see DESIGN.md); run with --gpus 1",
+            "is N/G trajectories per GPU with no exchange; run with --gpus 1",
             file=sys.stderr,
         )
         sys.exit(2)

@@ -9,7 +9,7 @@
  * (LaunchNumbaCudaAbi, :145-213), and the XLA FFI symbol that exposes the
  * launcher (modax_numba_cuda_abi_launch, :215-231). The entry points below
  * replace them one for one. An XLA FFI handler is a thin wrapper over
- * mdx_solve_launch (its binding is sketched in INTEGRATION.md); it is not
+ * mdx_solve_launch that reads its buffers and attributes and calls it; it is not
  * shipped, because no jaxlib with the FFI headers exists on the machines this
  * library is built and tested on.
  *
