@@ -146,11 +146,14 @@ int32_t mdx_cost_pilot_launch(int64_t handle, void* stream, int64_t n_traj, int3
                               double* cost);
 
 /* mdx_cost_pilot_launch that also leaves each trajectory's step state in
- * `resume`, (n_vars + 8) x n_traj doubles with the trajectory index fastest:
- * the state vector, then t, dt, the controller's two previous error norms, the
- * next save index and the step, accepted and rejected counts. The saves it
- * passed are already in `hist`. For kernels created with MDX_KERNEL_RESUMABLE
- * (the one-lane Rodas5P kernel). No counterpart in the reference. */
+ * `resume`, with the trajectory index fastest: for Rodas5P (n_vars + 8) x
+ * n_traj doubles, the state vector, then t, dt, the controller's two previous
+ * error norms, the next save index and the step, accepted and rejected counts;
+ * for Tsit5 (2 n_vars + 9) x n_traj, the state vector, its first-same-as-last
+ * stage, the same eight scalars and 1.0 when that stage holds (else 0.0). The
+ * saves it passed are already in `hist`. For kernels created with
+ * MDX_KERNEL_RESUMABLE (the one-lane Rodas5P and Tsit5 kernels). No
+ * counterpart in the reference. */
 int32_t mdx_cost_pilot_launch_resumable(int64_t handle, void* stream, int64_t n_traj,
                                         int32_t n_vars, int32_t n_params, int32_t n_save,
                                         int32_t layout, const double* y0, const double* times,
@@ -163,7 +166,8 @@ int32_t mdx_cost_pilot_launch_resumable(int64_t handle, void* stream, int64_t n_
  * the step state mdx_cost_pilot_launch_resumable left in `resume`, into the
  * same `hist`, so the pilot's steps are not taken again; the results are the
  * bits of a solve from y0. For kernels created with MDX_KERNEL_RESUMABLE
- * (one-lane or refill Rodas5P, no save hook). No counterpart in the reference. */
+ * (one-lane or refill Rodas5P or Tsit5, no save hook). No counterpart in the
+ * reference. */
 int32_t mdx_solve_launch_resumed(int64_t handle, void* stream, int32_t grid, int64_t n_traj,
                                  int32_t n_vars, int32_t n_params, int32_t n_save, int32_t layout,
                                  const double* y0, const double* times, const double* params,

@@ -393,11 +393,18 @@ def _params_in_regs(options, n_params):
     return n_params <= PARAMS_IN_REGS_MAX
 
 
+def resume_rows(method: str, n_vars: int) -> int:
+    """Doubles per trajectory of the step state a cost pilot leaves for the
+    solve that resumes it: the state and the eight step scalars (Rodas5P),
+    and for Tsit5 also its first-same-as-last stage and whether it holds."""
+    return n_vars + 8 if method == "rodas5p" else 2 * n_vars + 9
+
+
 def _register(code, block, unit_weights, plain_entry, refill_entry, hook=False, wave=False,
               resumable=False):
     """Handles of the one-lane-per-trajectory kernel and of the persistent
     refill kernel (0 when there is none: a wave-per-trajectory kernel);
-    ``resumable`` kernels take a cost pilot's step state (Rodas5P)."""
+    ``resumable`` kernels take a cost pilot's step state (``resume_rows``)."""
     flags = _runtime.KERNEL_UNIT_WEIGHTS if unit_weights else 0
     if hook:
         flags |= _runtime.KERNEL_HOOK
@@ -694,7 +701,8 @@ def tsit5_adjoint_kernel(ode_fn, n_vars: int, n_params: int, options: Tsit5Optio
 def tsit5_kernel(ode_fn, n_vars: int, n_params: int, options: Tsit5Options) -> Built:
     source, d = tsit5_source(ode_fn, n_vars, n_params, options)
     code = _runtime.code_object(source, f"tsit5_{getattr(ode_fn, '__name__', 'rhs')}.hip")
-    plain, refill = _register(code, d["block"], options.unit_weights, "mdx_tsit5", "mdx_tsit5_refill")
+    plain, refill = _register(code, d["block"], options.unit_weights, "mdx_tsit5", "mdx_tsit5_refill",
+                              resumable=True)
     return Built(
         method="tsit5", plain=plain, refill=refill, block=d["block"], n_vars=n_vars,
         n_params=n_params, unit_weights=options.unit_weights, source=source,

@@ -427,7 +427,7 @@ int32_t launch_impl(int64_t handle, void* stream, int32_t grid, int64_t n_traj, 
                    "without a save hook");
   if (resume && !k->resumable)
     return fail(1, "mdx_solve_launch: a step-state buffer is for kernels created with "
-                   "MDX_KERNEL_RESUMABLE (the one-lane and refill Rodas5P kernels)");
+                   "MDX_KERNEL_RESUMABLE (the one-lane and refill Rodas5P and Tsit5 kernels)");
   if (k->persistent && (!workspace || workspace_bytes < 8))
     return fail(1, "mdx_solve_launch: a persistent kernel needs a workspace of at least 8 bytes");
   if (n_traj == 0) return 0;

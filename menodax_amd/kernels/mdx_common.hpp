@@ -44,8 +44,9 @@ struct SolveArgs {
   u64* queue;             // persistent refill: next unclaimed trajectory
   const i64* order;       // persistent refill: claim order (a permutation), or null
   double* cost;           // cost pilot: work left per trajectory, written instead of saves
-  double* resume;         // step state per trajectory, (n_vars + 8, traj): a cost pilot
-                          // writes it, a solve given it starts each trajectory there
+  double* resume;         // step state per trajectory, (n_vars + 8, traj) for Rodas5P,
+                          // (2 n_vars + 9, traj) for Tsit5: a cost pilot writes it,
+                          // a solve given it starts each trajectory there
   double dt0, rtol, atol;
   i64 n_traj;
   i64 y0_ts, y0_vs;

@@ -219,6 +219,39 @@ __device__ __forceinline__ void init(State& s, const SolveArgs& a, i64 i) {
   s.accepted = 0;
   s.rejected = 0;
   s.has_fsal = false;
+#if !MDX_ADJOINT
+  if (a.resume && !a.cost) {
+    // where a cost pilot left the trajectory: the state, the first-same-as-last
+    // stage and the controller's memory, so the solve continues with the bits
+    // an unbroken one has (the pilot wrote the saves it passed). Layout
+    // [y | k7 | t dt err_prev err_prev2 save_idx n_steps accepted rejected
+    // has_fsal], a row of N trajectories per element.
+    const double* __restrict__ r = a.resume + i;
+    const i64 st = a.n_traj;
+#if MDX_ROLLED
+#pragma unroll 1
+    for (int j = 0; j < N; ++j) {
+      YJ(s, j) = r[j * st];
+      KQ(s, 6, j) = r[(N + j) * st];
+    }
+#else
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      s.y[j] = r[j * st];
+      s.k7[j] = r[(N + j) * st];
+    }
+#endif
+    s.t = r[(2 * N + 0) * st];
+    s.dt = r[(2 * N + 1) * st];
+    s.err_prev = r[(2 * N + 2) * st];
+    s.err_prev2 = r[(2 * N + 3) * st];
+    s.save_idx = (int)r[(2 * N + 4) * st];
+    s.n_steps = (int)r[(2 * N + 5) * st];
+    s.accepted = (int)r[(2 * N + 6) * st];
+    s.rejected = (int)r[(2 * N + 7) * st];
+    s.has_fsal = r[(2 * N + 8) * st] != 0.0;
+  }
+#endif
 }
 
 __device__ __forceinline__ bool active(const State& s, const SolveArgs& a) {
@@ -525,6 +558,34 @@ __device__ __forceinline__ void finish(State& s, const SolveArgs& a, i64 i) {
     // a cost pilot (max_steps is its step budget): the steps left at the
     // current step size instead of the saves and counters, 0 once finished
     a.cost[i] = (s.save_idx < a.n_save && s.t < s.tf) ? (s.tf - s.t) / s.dt : 0.0;
+    if (a.resume) {
+      // the step state the resumed solve starts from (layout as init reads it)
+      double* __restrict__ r = a.resume + i;
+      const i64 st = a.n_traj;
+#if MDX_ROLLED
+      MDX_LANE();
+#pragma unroll 1
+      for (int j = 0; j < N; ++j) {
+        r[j * st] = YJ(s, j);
+        r[(N + j) * st] = KQ(s, 6, j);
+      }
+#else
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        r[j * st] = s.y[j];
+        r[(N + j) * st] = s.k7[j];
+      }
+#endif
+      r[(2 * N + 0) * st] = s.t;
+      r[(2 * N + 1) * st] = s.dt;
+      r[(2 * N + 2) * st] = s.err_prev;
+      r[(2 * N + 3) * st] = s.err_prev2;
+      r[(2 * N + 4) * st] = (double)s.save_idx;
+      r[(2 * N + 5) * st] = (double)s.n_steps;
+      r[(2 * N + 6) * st] = (double)s.accepted;
+      r[(2 * N + 7) * st] = (double)s.rejected;
+      r[(2 * N + 8) * st] = s.has_fsal ? 1.0 : 0.0;
+    }
     return;
   }
   const double nan = __builtin_nan("");

@@ -23,7 +23,7 @@ from dataclasses import replace
 
 from . import _autodiff, _common, _kernels
 from ._kernels import Tsit5Options
-from .rodas5P import _tableau, finish, launch
+from .rodas5P import _tableau, _use_refill, finish, launch
 
 _SHARED_MAX_NVARS = 16
 
@@ -126,6 +126,9 @@ def solve(
         return state
     _check_backend(n_vars, backend)
     built = _kernels.tsit5_kernel(ode_fn, n_vars, n_params, options)
+    if launch_kw["order"] is None and refill is None and _use_refill(built, n):
+        # the resumed pilot's claim order, as for rodas5P.solve
+        launch_kw = dict(launch_kw, order="pilot")
     out = launch(built, y0_t.detach(), times, p_t.detach(), n,
                  weights=_common.error_weights_soa(error_weights, n, n_vars, dev), **launch_kw)
     return finish(out, return_stats, _common.wants_numpy(y0, params, t_span))

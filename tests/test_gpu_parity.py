@@ -146,6 +146,69 @@ def test_a_resumed_pilot_gives_the_unordered_solves_bits(lu):
         assert (st["loop_steps"] <= max_steps).all()
 
 
+@pytest.mark.parametrize("placement", [None, "rolled", "lds"])
+def test_a_resumed_tsit5_pilot_gives_the_unordered_solves_bits(placement):
+    """Tsit5's pilot leaves each trajectory's state, its first-same-as-last
+    stage and the controller's memory; the solve resumed from them gives the
+    unordered refill solve's bits and counters: trajectories that end inside
+    the 80-step pilot and after it, and a step budget inside and beyond the
+    pilot's length. Lorenz (registers), and a 24-variable Van der Pol ring in
+    the rolled step with its vectors in scratch and in LDS."""
+    import torch
+
+    from menodax_amd import _kernels, rodas5P as R5
+
+    rng = np.random.default_rng(9)
+    if placement is None:
+        fn, nv = S.lorenz, 3
+        y0, p = S.lorenz_rho_ensemble(300)
+        t = np.linspace(0.0, 1.5, 21)
+    else:
+        fn, nv = S.vdp_lattice(12, mu=1.0)[0], 24
+        y0 = rng.uniform(-2, 2, size=(300, nv))
+        p = rng.uniform(0.5, 3.0, size=(300, 1))
+        t = np.linspace(0.0, 1.6, 9)
+    built = _kernels.tsit5_kernel(fn, nv, 1, _kernels.Tsit5Options(placement=placement))
+    dev = torch.device("cuda")
+    args = (built, torch.tensor(y0, device=dev), torch.tensor(t, device=dev),
+            torch.tensor(p, device=dev), len(y0))
+    for max_steps in (100000, R5.PILOT_STEPS // 2, 3 * R5.PILOT_STEPS // 2):
+        kw = dict(dt0=0.0, rtol=1e-7, atol=1e-9, max_steps=max_steps)
+        base, bst = R5.finish(R5.launch(*args, refill=True, **kw), True, True)
+        hist, st = R5.finish(R5.launch(*args, order="pilot", **kw), True, True)
+        _same(hist, st, base, bst)
+        if max_steps == 100000:
+            steps = bst["loop_steps"]
+            assert (steps < R5.PILOT_STEPS).any() and (steps > R5.PILOT_STEPS).any()
+        else:
+            assert (st["loop_steps"] <= max_steps).all()
+
+
+def test_a_large_tsit5_ensemble_claims_in_the_resumed_pilot_order_by_default(monkeypatch):
+    """tsit5.solve, given no work_order, ranks an ensemble too large to be
+    resident at once by the resumed pilot, with the bits of index order."""
+    from menodax_amd import _kernels, _runtime
+
+    built = _kernels.tsit5_kernel(S.lorenz, 3, 1, _kernels.Tsit5Options())
+    n = _runtime.persistent_grid(built.refill) * built.block + 4096
+    y0, p = S.lorenz_rho_ensemble(n)
+    t = np.linspace(0.0, 1.0, 11)
+    kw = dict(rtol=1e-6, atol=1e-8, return_stats=True)
+    pilots = []
+    real = rodas5P._pilot_order
+
+    def spy(*a, **k):
+        pilots.append(a[4])
+        return real(*a, **k)
+
+    monkeypatch.setattr(rodas5P, "_pilot_order", spy)
+    hist, st = tsit5.solve(S.lorenz, y0, t, p, **kw)
+    assert pilots == [n]
+    ref, rst = tsit5.solve(S.lorenz, y0, t, p, refill=True, **kw)
+    assert pilots == [n]
+    _same(hist, st, ref, {k: np.asarray(v) for k, v in rst.items()})
+
+
 def test_max_steps_exhaustion_holds_the_last_state():
     y0, p = S.robertson_lognormal_ensemble(100, seed=4)
     t = S.robertson_benchmark_times()
