@@ -117,6 +117,18 @@ def test_a_step_state_buffer_needs_a_resumable_kernel(handles):
     assert b"null step-state" in lib.mdx_last_error()
 
 
+def test_the_step_state_rows_of_each_method():
+    """What a resumable pilot leaves per trajectory, as the kernels lay it out:
+    Rodas5P the state and eight step scalars, Tsit5 also its
+    first-same-as-last stage and whether that stage holds."""
+    from menodax_amd import _kernels
+
+    assert _kernels.resume_rows("rodas5p", 3) == 3 + 8
+    assert _kernels.resume_rows("rodas5p", 50) == 58
+    assert _kernels.resume_rows("tsit5", 3) == 2 * 3 + 9
+    assert _kernels.resume_rows("tsit5", 24) == 57
+
+
 def test_work_order_must_be_a_permutation():
     import torch
 
