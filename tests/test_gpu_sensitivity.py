@@ -143,6 +143,42 @@ def test_grad_matches_central_differences(method):
             assert abs(float(g[k]) - fd) <= 1e-5 * abs(fd)
 
 
+@pytest.mark.parametrize("method", ["rodas5p", "tsit5"])
+def test_a_joint_solve_resumed_from_its_pilot_gives_the_same_gradients(method):
+    """A differentiated solve with work_order="pilot" runs the joint [y, S]
+    kernel from the pilot's augmented step state: the history, the counters
+    and the gradients are the bits of the same joint solve in index order.
+    The trajectories run past the 80-step pilot (Robertson's joint solves take
+    122-146 steps), so they continue from its augmented state."""
+    import torch
+
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(12)
+    if method == "rodas5p":
+        solver, fn = rodas5P, S.robertson
+        y0, p = S.robertson_lognormal_ensemble(300, seed=7)
+        t = S.robertson_benchmark_times()
+        kw = dict(rtol=1e-6, atol=1e-8)
+    else:
+        solver, fn = tsit5, S.cascade
+        y0 = rng.uniform(0.5, 2.0, size=(300, 2))
+        p = 10.0 ** rng.uniform(-1.0, 2.3, size=(300, 2))
+        t = np.linspace(0.0, 2.0, 9)
+        kw = dict(rtol=1e-10, atol=1e-12)
+    out = []
+    for order in ({"refill": True}, {"work_order": "pilot"}):
+        yt = torch.tensor(y0, device=dev, requires_grad=True)
+        pt = torch.tensor(p, device=dev, requires_grad=True)
+        hist, st = solver.solve(fn, yt, t, pt, return_stats=True, **kw, **order)
+        (hist * hist).sum().backward()
+        out.append((hist.detach(), st, yt.grad, pt.grad))
+    (h0, s0, gy0, gp0), (h1, s1, gy1, gp1) = out
+    assert torch.equal(h0, h1) and torch.equal(gy0, gy1) and torch.equal(gp0, gp1)
+    for k in s0:
+        assert torch.equal(s0[k], s1[k]), k
+    assert bool((s0["loop_steps"] > rodas5P.PILOT_STEPS).any())
+
+
 def test_ensemble_gradients_are_per_trajectory_and_shared_params_sum():
     import torch
 
