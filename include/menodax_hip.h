@@ -17,7 +17,9 @@
  * failure, with the message available from mdx_last_error() on the calling
  * thread. Kernel handles are int64 keys into a process-lifetime registry, like
  * the reference's `function` attribute (:348-362). Nothing here allocates
- * device memory on a launch, and nothing synchronises the stream.
+ * device memory on a launch, and nothing synchronises the stream. A launch
+ * of zero trajectories validates its extents and returns; its buffers may be
+ * null (the data pointer of a zero-size allocation).
  */
 #ifndef MENODAX_HIP_H
 #define MENODAX_HIP_H

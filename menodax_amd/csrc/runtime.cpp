@@ -356,7 +356,7 @@ int32_t mdx_solve_launch_resumed(int64_t handle, void* stream, int32_t grid, int
                                  int32_t* rejected, int32_t* loop_steps, void* workspace,
                                  int64_t workspace_bytes, const int64_t* order,
                                  const double* resume) {
-  if (!resume) return fail(1, "mdx_solve_launch_resumed: null step-state buffer");
+  if (!resume && n_traj > 0) return fail(1, "mdx_solve_launch_resumed: null step-state buffer");
   return launch_impl(handle, stream, grid, n_traj, n_vars, n_params, n_save, layout, y0, times,
                      params, dt0, rtol, atol, max_steps, weights, hist, accepted, rejected,
                      loop_steps, nullptr, 0, workspace, workspace_bytes, order, nullptr,
@@ -369,7 +369,7 @@ int32_t mdx_cost_pilot_launch(int64_t handle, void* stream, int64_t n_traj, int3
                               double atol, int32_t steps, const double* weights, double* hist,
                               int32_t* accepted, int32_t* rejected, int32_t* loop_steps,
                               double* cost) {
-  if (!cost) return fail(1, "mdx_cost_pilot_launch: null cost buffer");
+  if (!cost && n_traj > 0) return fail(1, "mdx_cost_pilot_launch: null cost buffer");
   if (steps < 1) return fail(1, "mdx_cost_pilot_launch: steps must be positive");
   return launch_impl(handle, stream, 0, n_traj, n_vars, n_params, n_save, layout, y0, times,
                      params, dt0, rtol, atol, steps, weights, hist, accepted, rejected,
@@ -383,8 +383,9 @@ int32_t mdx_cost_pilot_launch_resumable(int64_t handle, void* stream, int64_t n_
                                         double atol, int32_t steps, const double* weights,
                                         double* hist, int32_t* accepted, int32_t* rejected,
                                         int32_t* loop_steps, double* cost, double* resume) {
-  if (!cost) return fail(1, "mdx_cost_pilot_launch_resumable: null cost buffer");
-  if (!resume) return fail(1, "mdx_cost_pilot_launch_resumable: null step-state buffer");
+  if (!cost && n_traj > 0) return fail(1, "mdx_cost_pilot_launch_resumable: null cost buffer");
+  if (!resume && n_traj > 0)
+    return fail(1, "mdx_cost_pilot_launch_resumable: null step-state buffer");
   if (steps < 1) return fail(1, "mdx_cost_pilot_launch_resumable: steps must be positive");
   return launch_impl(handle, stream, 0, n_traj, n_vars, n_params, n_save, layout, y0, times,
                      params, dt0, rtol, atol, steps, weights, hist, accepted, rejected,
@@ -406,13 +407,16 @@ int32_t launch_impl(int64_t handle, void* stream, int32_t grid, int64_t n_traj, 
                        ", n_vars=" + std::to_string(n_vars) + ", n_params=" +
                        std::to_string(n_params) + ", n_save=" + std::to_string(n_save) + ")");
   if (layout != 0 && layout != 1) return fail(1, "mdx_solve_launch: layout must be 0 or 1");
-  if (!y0 || !times || (!params && n_params > 0) || !hist || !accepted || !rejected ||
-      !loop_steps)
+  // an empty ensemble reads and writes nothing: its buffers may be null (the
+  // data pointer of a zero-size allocation)
+  if (n_traj > 0 && (!y0 || !times || (!params && n_params > 0) || !hist || !accepted ||
+                     !rejected || !loop_steps))
     return fail(1, "mdx_solve_launch: null buffer");
-  if (!weights && !k->unit_weights)
+  if (n_traj > 0 && !weights && !k->unit_weights)
     return fail(1, "mdx_solve_launch: this kernel reads error weights; weights is NULL");
-  if (hook_size > 0 && !hook_out) return fail(1, "mdx_solve_launch: hook_size > 0 needs hook_out");
-  if (k->hook && (hook_size <= 0 || !hook_out))
+  if (n_traj > 0 && hook_size > 0 && !hook_out)
+    return fail(1, "mdx_solve_launch: hook_size > 0 needs hook_out");
+  if (k->hook && (hook_size <= 0 || (n_traj > 0 && !hook_out)))
     return fail(1, "mdx_solve_launch: this kernel runs a save hook; hook_out and hook_size are required");
   if (max_steps < 0) return fail(1, "mdx_solve_launch: max_steps must be non-negative");
   if (k->adjoint) return fail(1, "mdx_solve_launch: an adjoint kernel is launched by mdx_adjoint_launch");
@@ -428,7 +432,7 @@ int32_t launch_impl(int64_t handle, void* stream, int32_t grid, int64_t n_traj, 
   if (resume && !k->resumable)
     return fail(1, "mdx_solve_launch: a step-state buffer is for kernels created with "
                    "MDX_KERNEL_RESUMABLE (the one-lane and refill Rodas5P and Tsit5 kernels)");
-  if (k->persistent && (!workspace || workspace_bytes < 8))
+  if (k->persistent && n_traj > 0 && (!workspace || workspace_bytes < 8))
     return fail(1, "mdx_solve_launch: a persistent kernel needs a workspace of at least 8 bytes");
   if (n_traj == 0) return 0;
 
@@ -551,10 +555,10 @@ int32_t adjoint_impl(int64_t handle, void* stream, int64_t n_traj, int32_t n_var
   if (layout != 0 && layout != 1) return fail(1, w + ": layout must be 0 or 1");
   if (ckpt_every < 1 || max_ckpt < 1)
     return fail(1, w + ": ckpt_every and max_ckpt must be positive");
-  if (!y0 || !times || (!params && n_params > 0) || !gbar || !grad_y0 ||
-      (!grad_p && n_params > 0) || !status || (!use_static && !workspace))
+  if (n_traj > 0 && (!y0 || !times || (!params && n_params > 0) || !gbar || !grad_y0 ||
+                     (!grad_p && n_params > 0) || !status || (!use_static && !workspace)))
     return fail(1, w + ": null buffer");
-  if (!weights && !k->unit_weights)
+  if (n_traj > 0 && !weights && !k->unit_weights)
     return fail(1, w + ": this kernel reads error weights; weights is NULL");
   if (max_steps < 0) return fail(1, w + ": max_steps must be non-negative");
   const int64_t need = mdx_adjoint_workspace_bytes(n_traj, n_vars, ckpt_every, max_ckpt);

@@ -149,6 +149,27 @@ def test_an_empty_ensemble_launches_nothing(handles):
     _launch(handles[0], n_traj=0)  # returns before any device call
 
 
+def test_an_empty_ensemble_may_pass_null_buffers(handles):
+    """Zero-size device arrays have null data pointers; with N = 0 every entry
+    point accepts them and returns before any device call, while N > 0 still
+    rejects them."""
+    nulls = dict(y0=0, times=0, params=0, weights=0, hist=0, accepted=0, rejected=0, loop_steps=0)
+    _launch(handles[0], n_traj=0, **nulls)
+    _launch(handles[1], n_traj=0, **nulls)  # persistent, and no workspace either
+    with pytest.raises(_runtime.NativeError, match="null buffer"):
+        _launch(handles[0], **nulls)
+    lib = _runtime.lib()
+    args = [handles[0], None, 0, 3, 3, 5, 1, 0, 0, 0, 0.0, 1e-6, 1e-8, 10, 0, 0, 0, 0, 0]
+    assert lib.mdx_cost_pilot_launch(*args, None) == 0
+    fn = S.vdp_lattice(1, mu=1.0, d=0.1, wrap=False)[0]
+    adj = _kernels.rodas5p_adjoint_kernel(fn, 2, 1, _kernels.KernelOptions(lu_precision="fp64"))
+    _runtime.adjoint_launch(adj.plain, None, 0, 2, 1, 20, 1, 0, 0, 0, 0.0, 1e-8, 1e-10, 100, 0,
+                            0, 0, 0, 0, 1, 1, 0, 0)
+    with pytest.raises(_runtime.NativeError, match="null buffer"):
+        _runtime.adjoint_launch(adj.plain, None, 4, 2, 1, 20, 1, 0, 0, 0, 0.0, 1e-8, 1e-10, 100,
+                                0, 0, 0, 0, 0, 1, 1, 0, 0)
+
+
 def test_kernel_create_validates():
     with pytest.raises(_runtime.NativeError, match="block must be"):
         _runtime.kernel_create(b"\x7fELF....", "x", 0, 0)

@@ -209,6 +209,23 @@ def test_a_large_tsit5_ensemble_claims_in_the_resumed_pilot_order_by_default(mon
     _same(hist, st, ref, {k: np.asarray(v) for k, v in rst.items()})
 
 
+def test_an_empty_ensemble_gives_empty_results():
+    """N = 0 through every entry: the solves (plain, in the pilot's order,
+    with sensitivities) and the adjoint return arrays with no trajectories."""
+    t = S.robertson_benchmark_times()
+    y0, p = np.zeros((0, 3)), np.zeros((0, 3))
+    for kw in (dict(), dict(work_order="pilot"), dict(refill=True)):
+        hist, st = rodas5P.solve(S.robertson, y0, t, p, return_stats=True, **kw)
+        assert hist.shape == (0, t.size, 3)
+        assert all(np.asarray(v).shape == (0,) for v in st.values())
+        hist = tsit5.solve(S.lorenz, y0, t[:5], np.zeros((0, 1)), **kw)
+        assert hist.shape == (0, 5, 3)
+    state, sens = rodas5P.solve_sensitivities(S.robertson, y0, t, p)
+    assert state.shape == (0, t.size, 3) and sens.shape == (0, t.size, 3, 6)
+    gy, gp = rodas5P.adjoint_gradients(S.robertson, y0, t, p, np.zeros((0, t.size, 3)))
+    assert gy.shape == (0, 3) and gp.shape == (0, 3)
+
+
 def test_max_steps_exhaustion_holds_the_last_state():
     y0, p = S.robertson_lognormal_ensemble(100, seed=4)
     t = S.robertson_benchmark_times()
