@@ -226,6 +226,36 @@ def test_an_empty_ensemble_gives_empty_results():
     assert gy.shape == (0, 3) and gp.shape == (0, 3)
 
 
+def test_a_single_save_time_is_the_initial_state():
+    """t_span of one time: the window is empty, the history is y0 and no step
+    is taken, in both methods and launch shapes, as the oracle has it."""
+    y0, p = S.robertson_lognormal_ensemble(100, seed=9)
+    t = np.array([0.0])
+    ref, rst = O.rodas5p(O.robertson(), y0, t, p, rtol=1e-6, atol=1e-8)
+    for refill in (False, True):
+        hist, st = rodas5P.solve(S.robertson, y0, t, p, rtol=1e-6, atol=1e-8,
+                                 return_stats=True, refill=refill)
+        _same(hist, st, ref, rst)
+        assert (st["loop_steps"] == 0).all()
+        np.testing.assert_array_equal(hist[:, 0], y0)
+        yl, pl = S.lorenz_rho_ensemble(100)
+        hist, st = tsit5.solve(S.lorenz, yl, t, pl, return_stats=True, refill=refill)
+        np.testing.assert_array_equal(hist[:, 0], yl)
+        assert (st["loop_steps"] == 0).all()
+
+
+def test_a_zero_step_budget_holds_the_initial_state():
+    """max_steps=0: every save holds y0, as the oracle has it, also when a
+    pilot order is asked for (the pilot then takes no step and ranks nothing)."""
+    y0, p = S.robertson_lognormal_ensemble(100, seed=9)
+    t = S.robertson_benchmark_times()
+    kw = dict(rtol=1e-6, atol=1e-8, max_steps=0, return_stats=True)
+    ref, rst = O.rodas5p(O.robertson(), y0, t, p, rtol=1e-6, atol=1e-8, max_steps=0)
+    for extra in (dict(), dict(refill=True), dict(work_order="pilot")):
+        hist, st = rodas5P.solve(S.robertson, y0, t, p, **kw, **extra)
+        _same(hist, st, ref, rst)
+
+
 def test_max_steps_exhaustion_holds_the_last_state():
     y0, p = S.robertson_lognormal_ensemble(100, seed=4)
     t = S.robertson_benchmark_times()
