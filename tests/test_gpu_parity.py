@@ -290,13 +290,6 @@ def test_partial_waves_and_broadcast_inputs(n):
     _same(hist, st, ref, rst)
 
 
-def test_a_single_save_time_takes_no_step():
-    y0, p = S.robertson_lognormal_ensemble(10, seed=7)
-    hist, st = rodas5P.solve(S.robertson, y0, np.array([0.0]), p, return_stats=True)
-    assert (st["loop_steps"] == 0).all()
-    np.testing.assert_array_equal(hist[:, 0, :], y0)
-
-
 def test_torch_inputs_give_torch_outputs_and_both_layouts_agree():
     import torch
 
