@@ -96,6 +96,10 @@ class KernelOptions:
     #: matrix and keep the stage vectors in FP64 (no counterpart in the
     #: reference, whose FP32 path stores the stage vectors in FP32)
     lu_refine: bool = False
+    #: the LDS adjoint also keeps its stage arguments in LDS and pulls each
+    #: stage's adjoint from the later ones when they fit at the workgroup size
+    #: (the same bits, fewer instructions), or never
+    adjoint_cache: bool = True
 
 
 #: the unrolled Rodas5P step keeps its tableau in vector registers up to this
@@ -185,11 +189,13 @@ def rodas5p_placement(n_vars: int, lu_precision: str, block: int, placement=None
     return defs
 
 
-def adjoint_lds_bytes(n_vars: int, n_lu: int, lu_precision: str) -> int:
+def adjoint_lds_bytes(n_vars: int, n_lu: int, lu_precision: str, cache: bool = False) -> int:
     """Per-lane LDS of the adjoint's LDS placement: the eight stage vectors (in
-    the LU's type), their eight adjoints and the factors' double copy."""
+    the LU's type), their eight adjoints and the factors' double copy; with
+    ``cache``, seven more double vectors (the stage arguments u_1..u_7, then
+    their adjoints)."""
     lu_bytes = 4 if lu_precision == "fp32" else 8
-    return 8 * n_vars * lu_bytes + 8 * n_vars * 8 + 8 * n_lu
+    return 8 * n_vars * lu_bytes + 8 * n_vars * 8 + 8 * n_lu + (7 * n_vars * 8 if cache else 0)
 
 
 def lu_size(n_vars: int, sparsity=None, ordering: str = "amd") -> int:
@@ -221,11 +227,12 @@ def adjoint_lds_supported(sparsity=None, ordering: str = "amd") -> bool:
 
 
 def adjoint_block(n_vars: int, n_lu: int, lu_precision: str, n_traj: int, n_cu: int):
-    """The rolled Rodas5P adjoint's (placement, workgroup size) for an
-    ensemble of ``n_traj``: the LDS placement at the widest of 64, 32 and 16
-    lanes whose LDS share holds the stage vectors, their adjoints and the
-    factors with the whole ensemble resident at once, else the scratch
-    ("rolled") step at 64 lanes. Residency decides because the reverse step
+    """The rolled Rodas5P adjoint's (placement, workgroup size, stage cache)
+    for an ensemble of ``n_traj``: the LDS placement at the widest of 64, 32
+    and 16 lanes whose LDS share holds the stage vectors, their adjoints and
+    the factors -- with the stage cache (``KernelOptions.adjoint_cache``) where
+    that still keeps the whole ensemble resident at once, else without it --
+    and otherwise the scratch ("rolled") step at 64 lanes. Residency decides because the reverse step
     waits on every access to these arrays: a small ensemble leaves SIMDs idle
     and gains from LDS latency, while a large one at a narrow LDS workgroup
     keeps fewer lanes resident than 64-lane scratch workgroups. The 20-variable
@@ -233,12 +240,13 @@ def adjoint_block(n_vars: int, n_lu: int, lu_precision: str, n_traj: int, n_cu: 
     405 -> 219 ms (LDS at 32 lanes), FP32 369 -> 224 ms (64 lanes); 65,536
     trajectories stay in scratch (907 / 786 ms against 908-1366 / 794-813 ms
     for the LDS forms)."""
-    per_lane = adjoint_lds_bytes(n_vars, n_lu, lu_precision)
-    for block in (64, 32, 16):
-        group = per_lane * block
-        if group <= LDS_BYTES_PER_CU and -(-n_traj // block) <= resident_groups(group, n_cu):
-            return "lds", block
-    return "rolled", DEFAULT_BLOCK
+    for cache in (True, False):
+        per_lane = adjoint_lds_bytes(n_vars, n_lu, lu_precision, cache)
+        for block in (64, 32, 16):
+            group = per_lane * block
+            if group <= LDS_BYTES_PER_CU and -(-n_traj // block) <= resident_groups(group, n_cu):
+                return "lds", block, cache
+    return "rolled", DEFAULT_BLOCK, False
 
 
 def resident_groups(lds_per_group: int, n_cu: int) -> int:
@@ -489,7 +497,7 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
     if adjoint and spec is not None:
         raise ValueError("the adjoint kernel differentiates the plain solve; drop the sensitivity spec")
     placement = options.placement
-    adjoint_lds = False
+    adjoint_lds = adjoint_cache = False
     if adjoint:
         # the adjoint reads the state from per-lane arrays: registers for a few
         # variables (every stage loop unrolled), the rolled step above, its
@@ -502,6 +510,8 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
             per_lane = adjoint_lds_bytes(n_vars, structure.size, options.lu_precision)
             adjoint_lds = (per_lane * block <= LDS_BYTES_PER_CU
                            and adjoint_lds_supported(options.sparsity, options.ordering))
+            adjoint_cache = adjoint_lds and options.adjoint_cache and adjoint_lds_bytes(
+                n_vars, structure.size, options.lu_precision, cache=True) * block <= LDS_BYTES_PER_CU
             placement = "rolled"
     # a dense matrix above the unrolled size lives in per-lane memory: LDS
     # when it fits (the forward step only)
@@ -511,7 +521,8 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
                               k_bytes=8 if options.lu_refine else None)
     if adjoint_lds:
         place = dict(place, MDX_LDS_K=1, placement="lds",
-                     lds_bytes=adjoint_lds_bytes(n_vars, structure.size, options.lu_precision) * block)
+                     lds_bytes=adjoint_lds_bytes(n_vars, structure.size, options.lu_precision,
+                                                 cache=adjoint_cache) * block)
     defs = {
         "MDX_N": n_vars,
         "MDX_NP": n_params,
@@ -540,6 +551,7 @@ def rodas5p_source(ode_fn, n_vars: int, n_params: int, options: KernelOptions, a
                               and not adjoint and spec is None) else 0,
         "MDX_REFINE": 1 if options.lu_refine else 0,
         "MDX_ADJ_LDS": 1 if adjoint_lds else 0,
+        "MDX_ADJ_CACHE": 1 if adjoint_cache else 0,
     }
     sens_src = ""
     if options.save_hook is not None or not options.save_history:

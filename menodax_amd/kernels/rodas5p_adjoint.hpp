@@ -58,10 +58,30 @@ namespace r5 {
 #if MDX_ADJ_LDS && !(MDX_ROLLED && MDX_LDS_K && !MDX_LDS_U && !MDX_LDS_Y && !MDX_LDS_LU)
 #error "the adjoint's LDS arrays go with the rolled step's LDS stage vectors only"
 #endif
+// MDX_ADJ_CACHE=1 (LDS placement only): seven more vectors a lane in LDS. The
+// step's recomputation leaves there each stage argument u_S (S = 1..7), which
+// the reverse sweep reads instead of re-forming it from the stage vectors, and
+// the sweep then stores each stage's u_bar_S in the same slot. Stage S's
+// k_bar_S is pulled when the sweep reaches it, from the r_bar_R and u_bar_R of
+// the stages R > S done before, in the order the pushing form adds them
+// (R = 7 first; its C term, then its u term), so the sums and their bits are
+// the same; the pushing form's read-modify-writes of every earlier k_bar at
+// every stage go.
+#ifndef MDX_ADJ_CACHE
+#define MDX_ADJ_CACHE 0
+#endif
+#if MDX_ADJ_CACHE && !MDX_ADJ_LDS
+#error "the adjoint's stage cache is part of its LDS placement"
+#endif
 
 #if MDX_ADJ_LDS
 __shared__ double lds_kb[8 * N * MDX_BLOCK];
 __shared__ double lds_lud[MDX_NLU * MDX_BLOCK];
+#if MDX_ADJ_CACHE
+// u_S of the step, then u_bar_S once stage S is reversed (S = 1..7)
+__shared__ double lds_us[7 * N * MDX_BLOCK];
+#define USV(S, j) lds_us[(((S) - 1) * N + (j)) * MDX_BLOCK + lane_]
+#endif
 // element e of this lane's row in an [element][lane] LDS array
 struct LdsRows {
   double* base;
@@ -160,6 +180,10 @@ __device__ void recompute(const double* __restrict__ y, double t, double h, PA& 
       for (int j = 0; j < N; ++j) u[j] = u[j] + (double)KV(S - 1, j);
       ts = t_end;
     }
+#if MDX_ADJ_CACHE
+    if (S > 0)
+      for (int j = 0; j < N; ++j) USV(S, j) = u[j];
+#endif
     double work[N];
     sys::rhs(u, ts, p, work);
     LUT r[N];
@@ -249,17 +273,45 @@ __device__ void reverse_step(const AdjointArgs& g, i64 i, PA& p, const double* _
       kd[j] = (double)KV(S, j);
       ub[j] = 0.0;
     }
+#if MDX_ADJ_CACHE
+    // k_bar_S: its saves' share, then every later stage's, stage 7 first (a
+    // stage's loads across the elements are independent, so they are in
+    // flight together)
+    for (int R = 7; R > S; --R) {
+      const double c = CM[R][S] * inv_dt;
+      for (int j = 0; j < N; ++j) rb[j] += c * KBV(R, j);  // KBV(R) holds r_bar_R
+      if (R >= 6) {
+        if (S == R - 1)
+          for (int j = 0; j < N; ++j) rb[j] += USV(R, j);  // USV(R) holds u_bar_R
+      } else {
+        const double a = AM[R - 1][S];
+        for (int j = 0; j < N; ++j) rb[j] += a * USV(R, j);
+      }
+    }
+#endif
     solve_t(lud, ipiv, rb);
     sys::jac_vjp(y, t, p, rb, kd, ybar, pbar);
     if (S < 5) {
       const double dd = h * DV[S];
       for (int j = 0; j < N; ++j) dTbar[j] += dd * rb[j];
     }
+#if MDX_ADJ_CACHE
+    for (int j = 0; j < N; ++j) KBV(S, j) = rb[j];
+    double ts;
+    if (S == 0) {
+      for (int j = 0; j < N; ++j) u[j] = y[j];
+      ts = t;
+    } else {
+      for (int j = 0; j < N; ++j) u[j] = USV(S, j);
+      ts = S < 5 ? t + CT[S - 1] * h : t + h;
+    }
+#else
     for (int q = 0; q < S; ++q) {
       const double c = CM[S][q] * inv_dt;
       for (int j = 0; j < N; ++j) KBV(q, j) += c * rb[j];
     }
     const double ts = stage_point(S, y, t, h, ks, u);
+#endif
     sys::f_vjp(u, ts, p, rb, ub, pbar);
     if (S >= 5) {
       for (int j = 0; j < N; ++j) ub[j] += ucarry[j];
@@ -268,13 +320,21 @@ __device__ void reverse_step(const AdjointArgs& g, i64 i, PA& p, const double* _
       // u_S = u_{S-1} + k_{S-1}
       for (int j = 0; j < N; ++j) {
         ucarry[j] = ub[j];
+#if MDX_ADJ_CACHE
+        USV(S, j) = ub[j];
+#else
         KBV(S - 1, j) += ub[j];
+#endif
       }
     } else if (S > 0) {
       // u_S = y + sum_q A_{S,q} k_q
       for (int j = 0; j < N; ++j) ybar[j] += ub[j];
+#if MDX_ADJ_CACHE
+      for (int j = 0; j < N; ++j) USV(S, j) = ub[j];
+#else
       for (int q = 0; q < S; ++q)
         for (int j = 0; j < N; ++j) KBV(q, j) += AM[S - 1][q] * ub[j];
+#endif
     } else {
       for (int j = 0; j < N; ++j) ybar[j] += ub[j];
     }

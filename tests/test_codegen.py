@@ -276,20 +276,24 @@ def test_a_small_sparse_ensemble_narrows_its_workgroup_to_keep_stage_vectors_in_
 
 def test_the_rolled_adjoint_keeps_its_stage_arrays_in_lds_while_the_ensemble_is_resident():
     A = _kernels.adjoint_block
-    # 20-variable chain, 67 factor elements: 3,096 B a lane in FP64, 2,188 in FP32
+    # 20-variable chain, 67 factor elements: 3,096 B a lane in FP64, 2,456 in
+    # FP32, and 1,120 more with the stage cache (seven double vectors)
     assert _kernels.adjoint_lds_bytes(20, 67, "fp64") == 8 * 20 * 8 * 2 + 8 * 67
     assert _kernels.adjoint_lds_bytes(20, 67, "fp32") == 8 * 20 * 4 + 8 * 20 * 8 + 8 * 67
-    assert A(20, 67, "fp64", 8192, 256) == ("lds", 32)
-    assert A(20, 67, "fp32", 8192, 256) == ("lds", 64)
-    # 8,193 FP64 trajectories need 257 groups of 32 (one 99 KB group a CU), then
-    # 16 lanes: three groups a CU
-    assert A(20, 67, "fp64", 8193, 256) == ("lds", 16)
-    assert A(20, 67, "fp64", 12288, 256) == ("lds", 16)
-    assert A(20, 67, "fp64", 12289, 256) == ("rolled", 64)
-    assert A(20, 67, "fp64", 65536, 256) == ("rolled", 64)
-    assert A(20, 67, "fp32", 65536, 256) == ("rolled", 64)
+    assert _kernels.adjoint_lds_bytes(20, 67, "fp64", cache=True) == 3096 + 7 * 20 * 8
+    assert A(20, 67, "fp64", 8192, 256) == ("lds", 32, True)
+    # the cache at 32 lanes keeps 8,192 resident, where 64 lanes without it would too
+    assert A(20, 67, "fp32", 8192, 256) == ("lds", 32, True)
+    # 8,193 FP64 trajectories need 257 groups of 32 (one group a CU with or
+    # without the cache) or 513 of 16 (two a CU with it): 16 lanes without it,
+    # three groups a CU
+    assert A(20, 67, "fp64", 8193, 256) == ("lds", 16, False)
+    assert A(20, 67, "fp64", 12288, 256) == ("lds", 16, False)
+    assert A(20, 67, "fp64", 12289, 256) == ("rolled", 64, False)
+    assert A(20, 67, "fp64", 65536, 256) == ("rolled", 64, False)
+    assert A(20, 67, "fp32", 65536, 256) == ("rolled", 64, False)
     # nothing fits at 16 lanes
-    assert A(200, 2000, "fp64", 10, 256) == ("rolled", 64)
+    assert A(200, 2000, "fp64", 10, 256) == ("rolled", 64, False)
     fn, _ = S.vdp_lattice(10, mu=1.0, d=0.1, wrap=False)
     sp = normalize_sparsity(S.vdp_sparsity(10, wrap=False), 20)
     assert _kernels.lu_size(20, sp) == 67 and _kernels.lu_size(20) == 400
@@ -297,8 +301,15 @@ def test_the_rolled_adjoint_keeps_its_stage_arrays_in_lds_while_the_ensemble_is_
         fn, 20, 1, _kernels.KernelOptions(lu_precision="fp64", sparsity=sp, placement="lds",
                                           trajectories_per_block=32), adjoint=True)
     assert "#define MDX_ADJ_LDS 1" in lds and "#define MDX_LDS_K 1" in lds
-    assert d["placement"]["lds_bytes"] == 3096 * 32
+    assert "#define MDX_ADJ_CACHE 1" in lds
+    assert d["placement"]["lds_bytes"] == 4216 * 32
     assert _runtime.rtc_compile(lds, "chain_adjoint_lds.hip")[:4] == b"\x7fELF"
+    plain, d = _kernels.rodas5p_source(
+        fn, 20, 1, _kernels.KernelOptions(lu_precision="fp64", sparsity=sp, placement="lds",
+                                          trajectories_per_block=32, adjoint_cache=False),
+        adjoint=True)
+    assert "#define MDX_ADJ_LDS 1" in plain and "#define MDX_ADJ_CACHE 0" in plain
+    assert d["placement"]["lds_bytes"] == 3096 * 32
     # table-driven sparse routines keep the scratch adjoint
     assert _kernels.adjoint_lds_supported(None) and _kernels.adjoint_lds_supported(sp)
     from menodax_amd import _sparse_direct as SD

@@ -149,8 +149,9 @@ def test_adjoint_placements_give_the_same_bits(lu):
 @pytest.mark.parametrize("lu,block", [("fp64", 32), ("fp64", 16), ("fp32", 64)])
 def test_the_chain_adjoint_in_lds_gives_the_scratch_kernels_bits(lu, block):
     # the 20-variable chain's rolled adjoint with its stage vectors, their
-    # adjoints and the factors in LDS, against the same step in scratch; 96
-    # trajectories leave the last workgroup partly empty
+    # adjoints and the factors in LDS (with the stage cache at 32 and 16 lanes,
+    # without it at 64 FP32 lanes, where it does not fit), against the same
+    # step in scratch; 96 trajectories leave the last workgroup partly empty
     import torch
 
     fn, y0, mu, t, target, kw = _vdp_problem(10, 96, seed=8)
@@ -169,7 +170,7 @@ def test_the_chain_adjoint_in_lds_gives_the_scratch_kernels_bits(lu, block):
     opts = rodas5P._adjoint_options(
         _kernels.KernelOptions(lu_precision=lu, sparsity=normalize_sparsity(kw["sparsity"], 20)),
         20, 96, None)
-    assert opts.placement == "lds" and opts.trajectories_per_block == (32 if lu == "fp64" else 64)
+    assert (opts.placement, opts.trajectories_per_block, opts.adjoint_cache) == ("lds", 32, True)
     dflt = rodas5P.adjoint_gradients(fn, y0, t, mu, g, ckpt_every=16, **kw)
     np.testing.assert_array_equal(dflt[0], ref[0])
     np.testing.assert_array_equal(dflt[1], ref[1])
