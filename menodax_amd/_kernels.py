@@ -237,9 +237,10 @@ def adjoint_block(n_vars: int, n_lu: int, lu_precision: str, n_traj: int, n_cu: 
     and gains from LDS latency, while a large one at a narrow LDS workgroup
     keeps fewer lanes resident than 64-lane scratch workgroups. The 20-variable
     Van der Pol chain (67 factor elements), 8,192 trajectories: FP64 LU
-    405 -> 219 ms (LDS at 32 lanes), 164 ms with the stage cache (the whole
-    gradient call); FP32 369 -> 224 ms (LDS at 64 lanes, without the cache);
-    65,536 trajectories stay in scratch (907 / 786 ms against 908-1366 /
+    405 -> 182-192 ms (LDS at 32 lanes), 164 ms with the stage cache; FP32
+    369 ms in scratch, 197-202 ms in LDS at 64 lanes without the cache,
+    178-187 ms with it at 32 (whole gradient calls); 65,536 trajectories stay
+    in scratch (907 / 786 ms against 908-1366 /
     794-813 ms for the LDS forms)."""
     for cache in (True, False):
         per_lane = adjoint_lds_bytes(n_vars, n_lu, lu_precision, cache)
