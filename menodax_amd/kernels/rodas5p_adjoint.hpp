@@ -274,9 +274,9 @@ __device__ void reverse_step(const AdjointArgs& g, i64 i, PA& p, const double* _
       ub[j] = 0.0;
     }
 #if MDX_ADJ_CACHE
-    // k_bar_S: its saves' share, then every later stage's, stage 7 first (a
-    // stage's loads across the elements are independent, so they are in
-    // flight together)
+    // k_bar_S: what the step's end and its saves gave it, then every later
+    // stage's share, stage 7 first (a stage's loads across the elements are
+    // independent, so they are in flight together)
     for (int R = 7; R > S; --R) {
       const double c = CM[R][S] * inv_dt;
       for (int j = 0; j < N; ++j) rb[j] += c * KBV(R, j);  // KBV(R) holds r_bar_R
